@@ -1,0 +1,38 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+    config.addinivalue_line("markers", "slow: long-running CPU test")
+
+
+def random_even(rng, shape, stats, encoder="canonical"):
+    """Random Grassmann tensor with the odd-total-parity entries removed (what gtn.random does with
+    trimming, __init__.py:6021-6045), built from a seeded generator."""
+    from oracle import gtn_oracle as O
+    X = rng.standard_normal(shape)
+    par = np.zeros(shape, dtype=np.int64)
+    for ax, (d, s) in enumerate(zip(shape, stats)):
+        if s in (1, -1):
+            p = O.parity_of_canonical(np.arange(d))
+            sh = [1] * len(shape)
+            sh[ax] = d
+            par = par + p.reshape(sh)
+    X = X * (par % 2 == 0)
+    T = O.dense(X, statistics=stats)
+    if encoder == "parity-preserving":
+        T = T.switch_encoder()
+    return T
+
+
+@pytest.fixture
+def rng():
+    return np.random.default_rng(1234)

@@ -1,0 +1,93 @@
+"""CPU tests that pin the oracle's join/split, svd, eig, hconjugate and TRG (no GPU)."""
+import numpy as np
+import pytest
+
+from oracle import gtn_oracle as O
+from conftest import random_even
+
+
+@pytest.mark.parametrize("enc", ["canonical", "parity-preserving"])
+def test_join_split_roundtrip(rng, enc):
+    A = random_even(rng, (4, 2, 3, 8), (1, -1, 0, -1), enc)
+    M = O.join_legs(A, "(ij)(kl)", intermediate_stat=(1, -1))
+    assert M.shape == (8, 24) and M.statistics == (1, "*") and M.encoder == "parity-preserving"
+    B = O.split_legs(M, "(ij)(kl)", (1, -1, 0, -1), (4, 2, 3, 8), intermediate_stat=(1, -1))
+    B = B.force_encoder(enc)
+    np.testing.assert_array_equal(A.data, B.data)
+
+
+def test_join_is_c_order_reshape_with_grouping_sign(rng):
+    """Fermionic join = grouping sign (-1)^p(a) on +1 legs of a -1 group, then C-order reshape
+    (first leg = high bits), then the encoder on the fused index (__init__.py:2947-3068)."""
+    A = random_even(rng, (4, 8, 2, 4), (1, 1, -1, -1))
+    M = O.join_legs(A, "(ij)(kl)", intermediate_stat=(-1, 1))
+    p = O.parity_of_canonical
+    s = (1 - 2 * p(np.arange(4)))[:, None, None, None] * (1 - 2 * p(np.arange(8)))[None, :, None, None]
+    ref = (A.data * s).reshape(32, 8)
+    ref = ref[O.encoder(np.arange(32))][:, O.encoder(np.arange(8))]
+    np.testing.assert_array_equal(M.data, ref)
+
+
+def test_switch_encoder_and_format_are_involutions(rng):
+    A = random_even(rng, (4, 8, 3), (1, -1, 0))
+    np.testing.assert_array_equal(A.switch_encoder().switch_encoder().data, A.data)
+    np.testing.assert_array_equal(A.switch_format().switch_format().data, A.data)
+    np.testing.assert_array_equal(A.switch_parity().switch_parity().data, A.data)
+
+
+def test_svd_reconstruction_and_blocks(rng):
+    A = random_even(rng, (4, 4, 4), (1, 1, -1))
+    U, S, V = A.svd("i|jk")
+    USV = O.einsum("ia,ab,bjk->ijk", U, S, V)
+    assert (A - USV).norm < 1e-13
+    # singular values equal numpy's on the two parity blocks of the joined matrix
+    M = O.join_legs(A, "(i)(jk)", "matrix", intermediate_stat=(-1, 1)).force_encoder("parity-preserving")
+    ME, MO = O.parity_blocks(M.data)
+    ref = np.sort(np.concatenate([np.linalg.svd(ME, compute_uv=False), np.linalg.svd(MO, compute_uv=False)]))
+    got = np.sort(np.abs(np.diag(S.force_format("matrix").data)))
+    got = got[got > 1e-12]
+    np.testing.assert_allclose(got, ref[ref > 1e-12], rtol=1e-12)
+
+
+def test_svd_truncation_and_padding(rng):
+    """int(chi/2) per parity block, then both blocks padded to the next power of two."""
+    A = random_even(rng, (8, 8, 8, 8), (1, -1, -1, 1))
+    for chi, bond in ((24, 32), (16, 16), (10, 16), (6, 8), (4, 4)):
+        U, S, V = A.svd("ab|cd", chi)
+        assert U.shape == (8, 8, bond) and S.shape == (bond, bond) and V.shape == (bond, 8, 8)
+        nz = np.count_nonzero(np.abs(np.diag(S.data)) > 0)
+        assert nz == 2 * (chi // 2)
+
+
+def test_hconjugate_and_eig_doc_identities(rng):
+    A = random_even(rng, (4, 4, 4), (1, 1, -1))
+    cA = A.hconjugate("i|jk")
+    assert cA.statistics == (-1, 1, -1)
+    H = O.einsum("ijk,jkI->iI", A, cA)
+    assert (H - H.hconjugate("i|I")).norm < 1e-14
+    H2 = O.einsum("jki,iJK->jkJK", A.hconjugate("i|jk"), A)
+    U, S, V = H2.eig("jk|JK")
+    assert (U - V.hconjugate("a|JK")).norm < 1e-13
+    assert (H2 - O.einsum("jka,ab,bJK->jkJK", U, S, V)).norm < 1e-12
+
+
+def test_trg_error_identity_at_full_bond(rng):
+    """At a bond that truncates nothing, Z(T') equals Z(T,T) (gauge2d.py:1737-1740)."""
+    T = O.site_tensor(4, seed=5)
+    _, nrm, err = O.trg(T, dcut=16, error_test=True)
+    assert err < 1e-12 and nrm > 0
+
+
+def test_trg_literal_equals_sector():
+    T = O.site_tensor(4, seed=6)
+    a, na = O.trg(T, dcut=4, mode="literal")
+    b, nb = O.trg(T, dcut=4, mode="sector")
+    assert abs(na - nb) < 1e-12 * na
+    np.testing.assert_allclose(a.data, b.data, atol=1e-12)
+
+
+def test_site_tensor_generator():
+    T = O.site_tensor(16, seed=0)
+    assert T.shape == (16,) * 4 and T.encoder == "parity-preserving" and T.statistics == (1, 1, -1, -1)
+    assert abs(np.abs(T.data).max() - 1.0) < 1e-15
+    assert O.is_grassmann_even(T)
