@@ -1,0 +1,1101 @@
+"""Grassmann tensors on the GPU and the operations of grassmanntn's hot path.
+
+Mirrors the reference's Python API (dense tensors with statistics / encoder / format, einsum,
+join_legs / split_legs, svd / eig with '(ij)(kl)' grouping, hconjugate, sqrt / power).  Tensor data
+lives in HBM as DeviceArrays owned by libgtn_hip.so; every numeric step runs in its HIP kernels.
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+
+from . import _lib as L
+from ._device import DeviceArray, copy2d
+from ._perm import Perm, strides
+from ._plan import (BOSE, FERMI, HYBRID, SEPARATORS, Contraction, GrassmannError, denumerate, fail,
+                    fresh_char, group_info, parse_partition, sign_table)
+
+ENCODERS = ("canonical", "parity-preserving")
+FORMATS = ("standard", "matrix")
+ALLOWED_STAT = (0, 1, -1, HYBRID)
+NUMER_CUTOFF = 1.0e-14
+
+
+def _pkg_flag(name, default):
+    pkg = sys.modules.get("grassmanntn_amd")
+    return getattr(pkg, name, default) if pkg is not None else default
+
+
+# --------------------------------------------------------------------------------------------------
+# precision of the contraction GEMMs
+# --------------------------------------------------------------------------------------------------
+class _Settings:
+    precision = "f64"       # GEMM arithmetic of einsum: f64 | f32 | bf16x3 | bf16x6
+    svd_precision = "f64"   # range-finder GEMMs of the block SVD
+    svd_oversample = 96
+    svd_iters = 10
+    seed = 12345
+
+
+settings = _Settings()
+
+
+class precision:
+    """Context manager: ``with precision('bf16x3'): ...`` sets the contraction GEMM arithmetic."""
+
+    def __init__(self, gemm="f64", svd=None):
+        if gemm not in L.PREC or (svd is not None and svd not in L.PREC):
+            raise ValueError("precision must be one of %s" % list(L.PREC))
+        self.gemm, self.svd = gemm, svd
+
+    def __enter__(self):
+        self._old = (settings.precision, settings.svd_precision)
+        settings.precision = self.gemm
+        if self.svd is not None:
+            settings.svd_precision = self.svd
+        return self
+
+    def __exit__(self, *a):
+        settings.precision, settings.svd_precision = self._old
+
+
+def _to_device(x):
+    if isinstance(x, DeviceArray):
+        return x.clone()
+    a = np.asarray(x)
+    if a.dtype.kind == "c":
+        a = a.astype(np.complex128)
+    elif a.dtype not in (np.float64, np.float32):
+        a = a.astype(np.float64)
+    if a.ndim == 0:
+        a = a.reshape(1)
+    return DeviceArray.from_numpy(a)
+
+
+def _is_pp(T, axis):
+    return T.encoder == "parity-preserving" and T.statistics[axis] in FERMI
+
+
+# --------------------------------------------------------------------------------------------------
+# dense (__init__.py:814-1195)
+# --------------------------------------------------------------------------------------------------
+class dense:
+    """Dense Grassmann tensor: coefficient array in HBM plus per-leg statistics (+1 fermion,
+    -1 conjugated fermion, 0 boson, '*' hybrid), encoder and format (__init__.py:818-870)."""
+
+    __array_priority__ = 1000000
+
+    def __init__(self, data=None, encoder="canonical", format="standard", statistics=None):
+        if encoder not in ENCODERS:
+            fail("Error[dense]: Unknown encoder.")
+        if format not in FORMATS:
+            fail("Error[dense]: Unknown format.")
+        self.data = None
+        self.statistics = None
+        self.encoder = encoder
+        self.format = format
+        self._even = None
+        given = True
+        if isinstance(data, dense):
+            self.data = data.data.clone()
+            self.statistics = data.statistics
+            self.encoder = data.encoder
+            self.format = data.format
+            self._even = data._even
+        elif isinstance(data, sparse):
+            fail("Error[dense]: the sparse type is not provided by this build.")
+        elif data is None:
+            given = False
+        else:
+            from .block import block
+            if isinstance(data, block):
+                T = data.todense()
+                self.data, self.statistics, self.encoder, self.format = T.data, T.statistics, T.encoder, T.format
+            else:
+                self.data = _to_device(data)
+        if statistics is not None:
+            self.statistics = tuple(statistics) if not np.isscalar(statistics) else (statistics,)
+        if given and not _pkg_flag("skip_power_of_two_check", False):
+            for i, d in enumerate(self.data.shape):
+                if self.statistics[i] in FERMI and d != 2 ** int(math.floor(math.log2(d))):
+                    fail("Error[dense]: Some of the fermionic tensor shapes are not a power of two."
+                         "\n              Have you added the <statistics> argument when calling this function?")
+
+    # ---- array-like properties ----------------------------------------------------------------
+    def __getitem__(self, index):
+        return self.data.numpy()[index]
+
+    def __setitem__(self, index, value):
+        h = self.data.numpy()
+        h[index] = value
+        self.data = DeviceArray.from_numpy(h)
+        self._even = None
+
+    shape = property(lambda self: tuple(self.data.shape))
+    size = property(lambda self: self.data.numel())
+    ndim = property(lambda self: self.data.ndim)
+
+    @property
+    def norm(self):
+        return math.sqrt(_sumsq(self.data))
+
+    @property
+    def nnz(self):
+        return int((np.abs(self.data.numpy()) > _pkg_flag("numer_display_cutoff", 1000 * NUMER_CUTOFF)).sum())
+
+    def numpy(self):
+        return self.data.numpy()
+
+    def copy(self):
+        r = dense.__new__(dense)
+        r.data = self.data.clone()
+        r.statistics, r.encoder, r.format, r._even = self.statistics, self.encoder, self.format, self._even
+        return r
+
+    def _like(self, data, statistics=None, encoder=None, format=None, even=None):
+        r = dense.__new__(dense)
+        r.data = data
+        r.statistics = self.statistics if statistics is None else tuple(statistics)
+        r.encoder = self.encoder if encoder is None else encoder
+        r.format = self.format if format is None else format
+        r._even = even
+        return r
+
+    # ---- arithmetic (__init__.py:933-1000) -----------------------------------------------------
+    def __add__(self, other):
+        if (self.shape != other.shape or self.statistics != other.statistics or self.format != other.format
+                or self.encoder != other.encoder):
+            fail("Error[dense.+]: Inconsistent object properties")
+        out = DeviceArray.empty(self.data.shape, self.data.dtype)
+        _axpby(self.data, other.data, out, 1.0, 1.0)
+        ev = True if (self._even and other._even) else None
+        return self._like(out, even=ev)
+
+    def __mul__(self, other):
+        if not np.isscalar(other):
+            fail("Error[dense.*]: Only scalar multiplication is allowed.")
+        data = self.data
+        if isinstance(other, complex) and not data.is_complex():
+            data = data.astype_complex()
+        out = DeviceArray.empty(data.shape, data.dtype)
+        _axpby(data, None, out, other, 0.0)
+        return self._like(out, even=self._even)
+
+    def __truediv__(self, other):
+        if not np.isscalar(other):
+            fail("Error[dense./]: Only scalar division is allowed.")
+        return self * (1.0 / other)
+
+    def __pos__(self):
+        return self
+
+    def __neg__(self):
+        return self * (-1)
+
+    def __radd__(self, other):
+        return self + other
+
+    def __sub__(self, other):
+        return self + (-1) * other
+
+    def __rsub__(self, other):
+        return other + (-1) * self
+
+    def __rmul__(self, other):
+        return self * other
+
+    def __len__(self):
+        return len(self.data)
+
+    def __str__(self):
+        return str(self.numpy())
+
+    def __repr__(self):
+        return repr(self.numpy())
+
+    def info(self, name=None):
+        print()
+        if name is not None:
+            print("        name:", name)
+        print("  array type: dense")
+        print("       shape:", self.shape)
+        print("  statistics:", self.statistics)
+        print("      format:", self.format)
+        print("     encoder:", self.encoder)
+        print("        norm:", self.norm)
+        print()
+
+    display = info
+
+    # ---- encoder / format / parity switches (__init__.py:1011-1170) ----------------------------
+    def switch_encoder(self, save_memory=False):
+        enc = "parity-preserving" if self.encoder == "canonical" else "canonical"
+        out = _relayout(self, list(range(self.ndim)), out_encoder=enc)
+        return self._like(out, encoder=enc, even=self._even)
+
+    def switch_format(self, save_memory=False):
+        for s in self.statistics:
+            if s == HYBRID:
+                fail("Error[switch_format]: Cannot switch format with a hybrid index.\n"
+                     "                      Split them into bosonic and fermionic ones first!")
+        sig = [s == -1 for s in self.statistics]
+        out = _relayout(self, list(range(self.ndim)), out_encoder=self.encoder, sigma=sig)
+        fmt = "matrix" if self.format == "standard" else "standard"
+        return self._like(out, format=fmt, even=self._even)
+
+    def switch_parity(self, save_memory=False):
+        for s in self.statistics:
+            if s == HYBRID:
+                fail("Error[switch_parity]: Cannot switch format with a hybrid index.\n"
+                     "                      Split them into bosonic and fermionic ones first!")
+        lin = [s == 1 for s in self.statistics]
+        out = _relayout(self, list(range(self.ndim)), out_encoder=self.encoder, parity_lin=lin)
+        return self._like(out, even=self._even)
+
+    def force_encoder(self, target="canonical"):
+        if target not in ENCODERS:
+            fail("Error[dense.force_encoder]: Unrecognized target encoder.")
+        return self.switch_encoder() if target != self.encoder else self.copy()
+
+    def force_format(self, target="standard"):
+        if target not in FORMATS:
+            fail("Error[dense.force_format]: Unrecognized target format.")
+        return self.switch_format() if target != self.format else self.copy()
+
+    def is_even(self):
+        """Grassmann evenness (cached): no entry of odd total fermionic parity is nonzero."""
+        if self._even is None:
+            self._even = is_grassmann_even(self)
+        return self._even
+
+    # ---- methods that forward to functions -----------------------------------------------------
+    def join_legs(self, string_inp, make_format="standard", intermediate_stat=None, save_memory=False):
+        return join_legs(self, string_inp, make_format, intermediate_stat)
+
+    def split_legs(self, string_inp, final_stat, final_shape, intermediate_stat=None, save_memory=False):
+        return split_legs(self, string_inp, final_stat, final_shape, intermediate_stat)
+
+    def hconjugate(self, input_string, save_memory=False):
+        return hconjugate(self, input_string)
+
+    def svd(self, string_inp, cutoff=None, save_memory=False):
+        return svd(self, string_inp, cutoff)
+
+    def eig(self, string_inp, cutoff=None, debug_mode=False, save_memory=False):
+        return eig(self, string_inp, cutoff)
+
+    def toblock(self):
+        from .block import block
+        return block(self)
+
+
+class sparse:
+    """Type placeholder so that ``type(T) == grassmanntn_amd.sparse`` checks work (the reference's
+    sparse COO tensors, __init__.py:1196-1478, are not provided by this build)."""
+
+    def __init__(self, *a, **k):
+        raise NotImplementedError("sparse Grassmann tensors are not provided by this build; use dense")
+
+
+# --------------------------------------------------------------------------------------------------
+# device helpers
+# --------------------------------------------------------------------------------------------------
+def _sumsq(t):
+    lib = L.lib()
+    out = DeviceArray.empty(1, np.float64)
+    L.check(lib.gtn_sumsq(L.ptr(t), t.numel(), L.dtype_code(t), L.ptr(out), L.stream_ptr()), "gtn_sumsq")
+    return float(out.item())
+
+
+def _axpby(x, y, out, a, b):
+    lib = L.lib()
+    a, b = complex(a), complex(b)
+    L.check(lib.gtn_axpby(L.ptr(x), L.ptr(y) if y is not None else None, L.ptr(out), x.numel(), L.dtype_code(x),
+                          a.real, a.imag, b.real, b.imag, L.stream_ptr()), "gtn_axpby")
+
+
+def is_grassmann_even(T):
+    """(__init__.py:3900-3925) True iff every entry of odd total fermionic parity is exactly zero."""
+    from .block import block
+    if isinstance(T, block):
+        return T.is_even()
+    lib = L.lib()
+    nd = T.ndim
+    dims = (C.c_int64 * max(nd, 1))(*T.shape)
+    fer = (C.c_int32 * max(nd, 1))(*[1 if s in FERMI else 0 for s in T.statistics])
+    pp = (C.c_int32 * max(nd, 1))(*[1 if T.encoder == "parity-preserving" else 0] * nd)
+    out = DeviceArray.empty(1, np.float64)
+    L.check(lib.gtn_odd_norm(L.ptr(T.data), nd, dims, fer, pp, L.dtype_code(T.data), L.ptr(out), L.stream_ptr()),
+            "gtn_odd_norm")
+    return float(out.item()) == 0.0
+
+
+def _relayout(T, order, out_encoder, sigma=None, parity_lin=None, table_chars=None, codes=None, alpha=1.0):
+    """out axis a = T axis order[a], in `out_encoder`; sigma[a]: multiply sigma(canonical index) of
+    source axis a; parity_lin[a]: multiply (-1)^p of source axis a."""
+    P = Perm()
+    ins = []
+    st = strides(T.shape)
+    for a, d in enumerate(T.shape):
+        ins.append(P.add_in(d, st[a], enc=_is_pp(T, a)))
+    bits = {}
+    lin_axes = [a for a in range(T.ndim) if parity_lin and parity_lin[a]]
+    for k, a in enumerate(lin_axes):
+        bits[a] = k
+    for a in order:
+        d = T.shape[a]
+        pp = out_encoder == "parity-preserving" and T.statistics[a] in FERMI
+        v = P.add_virt(d, enc=pp)
+        P.add_comp(v, d, [(ins[a], 1)], bit=bits.get(a, -1), sigma=bool(sigma and sigma[a]))
+        P.add_out(d, [(v, 1)])
+    if lin_axes:
+        n = len(lin_axes)
+        P.set_table([bin(m).count("1") & 1 for m in range(1 << n)])
+    return P.run(T.data, alpha=alpha)
+
+
+# --------------------------------------------------------------------------------------------------
+# einsum (__init__.py:2308-2344 dispatcher; einsum_ds :1633-2306)
+# --------------------------------------------------------------------------------------------------
+def einsum(*args, ignore_anticommutation=False):
+    subscripts = denumerate(args[0].replace(" ", ""))
+    if subscripts.count("->") > 1:
+        fail("Error[einsum]: There should be no more than one '->' indicator in the subscript input.")
+    inp = subscripts.split("->")[0]
+    nobj = len(inp.split(","))
+    ops = list(args[1:1 + nobj])
+    from .block import block, einsum_block
+    if isinstance(ops[0], block):
+        return einsum_block(subscripts, *ops, ignore_anticommutation=ignore_anticommutation)
+    ops = [o if isinstance(o, dense) else dense(o) for o in ops]
+    first = ops[0]
+    if nobj >= 3:
+        return _einsum_pairwise(subscripts, ops, ignore_anticommutation)
+    plan = Contraction(subscripts, [(o.statistics, o.shape) for o in ops])
+    if nobj == 1:
+        res = _einsum1(plan, ops[0], first, ignore_anticommutation)
+    else:
+        res = _einsum2(plan, ops[0], ops[1], first, ignore_anticommutation)
+    return res
+
+
+def _einsum_pairwise(subscripts, ops, ignore):
+    """Contract the first two operands into an intermediate whose legs keep their order of
+    appearance, then recurse (Grassmann contraction is associative under this ordering)."""
+    inp, out = (subscripts.split("->") + [""])[:2] if "->" in subscripts else (subscripts, None)
+    strs = inp.split(",")
+    s1, s2, rest = strs[0], strs[1], strs[2:]
+    later = "".join(rest) + (out or "")
+    keep = []
+    for c in s1 + s2:
+        if c in keep:
+            continue
+        n12 = (s1 + s2).count(c)
+        if c in later or n12 == 1:
+            keep.append(c)
+    inter = "".join(keep)
+    mid = einsum(s1 + "," + s2 + "->" + inter, ops[0], ops[1], ignore_anticommutation=ignore)
+    sub2 = ",".join([inter] + rest) + ("->" + out if out is not None else "")
+    return einsum(sub2, mid, *ops[2:], ignore_anticommutation=ignore)
+
+
+def _fermionic(plan, c):
+    return any(t in FERMI for t in plan.stat_of[c])
+
+
+def _prod(xs):
+    r = 1
+    for x in xs:
+        r *= x
+    return r
+
+
+def _out_encoder_pp(first, stat):
+    return first.encoder == "parity-preserving" and stat in FERMI
+
+
+def _einsum1(plan, A, first, ignore):
+    s = plan.inputs[0]
+    out = plan.output
+    dims = plan.dims
+    summed = [c for c in dict.fromkeys(s) if c not in out]
+    fa = [c for c in dict.fromkeys(s) if _fermionic(plan, c)]
+    lin, quad = ({}, {}) if ignore else plan.quadratic_form()
+    codes = sign_table(fa, lin, quad)
+    P = Perm()
+    st = strides(A.shape)
+    ins = [P.add_in(A.shape[a], st[a], enc=_is_pp(A, a), sigma=(A.format == "matrix" and A.statistics[a] == -1))
+           for a in range(A.ndim)]
+    where = {}
+    for a, c in enumerate(s):
+        where.setdefault(c, []).append(ins[a])
+    pairs = set(plan.pairs) if not ignore else set()
+    for c in list(out) + summed:
+        tg = [(i, 1) for i in where[c]]
+        if len(tg) > 2:
+            raise NotImplementedError("index '%s' repeated more than twice on one operand" % c)
+        pp = c in out and _out_encoder_pp(first, plan.stat_of[c][0])
+        sig_out = c in out and first.format == "matrix" and plan.stat_of[c][0] == -1
+        v = P.add_virt(dims[c], enc=pp, sigma=sig_out)
+        P.add_comp(v, dims[c], tg, bit=fa.index(c) if c in fa else -1, sigma=c in pairs)
+        P.add_out(dims[c], [(v, 1)])
+    P.set_table(codes)
+    g = P.run(A.data)
+    red = _prod(dims[c] for c in summed)
+    if summed:
+        rows = g.numel() // red
+        r = DeviceArray.empty(rows, g.dtype)
+        L.check(L.lib().gtn_reduce_last(L.ptr(g), L.ptr(r), rows, red, L.dtype_code(g), L.stream_ptr()), "reduce")
+        g = r.reshape(tuple(dims[c] for c in out))
+    if not plan.has_output:
+        return g.numpy().reshape(-1)[0].item()
+    res = dense.__new__(dense)
+    res.data = g.reshape(tuple(dims[c] for c in out))
+    res.statistics = plan.out_stats
+    res.encoder, res.format = first.encoder, first.format
+    res._even = True if A._even else None
+    return res
+
+
+def _gemm(groups, dtype_code, prec):
+    lib = L.lib()
+    arr = (L.GemmDesc * len(groups))(*groups)
+    L.check(lib.gtn_gemm_grouped(arr, len(groups), dtype_code, L.PREC[prec], L.stream_ptr()), "gtn_gemm_grouped")
+
+
+def _gemm_desc(A, sam, sak, B, sbk, sbn, Cp, scm, scn, M, N, K, batch=1, sab=0, sbb=0, scb=0, alpha=1.0, beta=0.0):
+    d = L.GemmDesc()
+    d.A, d.B, d.C = A, B, Cp
+    d.M, d.N, d.K, d.batch = M, N, K, batch
+    d.sam, d.sak, d.sbk, d.sbn, d.scm, d.scn = sam, sak, sbk, sbn, scm, scn
+    d.sab, d.sbb, d.scb = sab, sbb, scb
+    d.alpha, d.beta = alpha, beta
+    return d
+
+
+def _operand_gather(T, s, plan, layout, fchars, codes, sig_chars):
+    """Gather operand T (index string s) into the GEMM layout.
+
+    layout: list of raw axes, each ('b', char) bosonic/batch leg, ('P',) parity, ('F', group) the
+    half (blocked) or full fused fermionic index of a group; groups: dict name -> list of chars."""
+    P = Perm()
+    st = strides(T.shape)
+    ins = [P.add_in(T.shape[a], st[a], enc=_is_pp(T, a), sigma=(T.format == "matrix" and T.statistics[a] == -1))
+           for a in range(T.ndim)]
+    where = {}
+    for a, c in enumerate(s):
+        where.setdefault(c, []).append(ins[a])
+    dims = plan.dims
+    virt_of = {}
+    for item in layout["axes"]:
+        if item[0] == "b":
+            c = item[1]
+            v = P.add_virt(dims[c])
+            P.add_comp(v, dims[c], [(i, 1) for i in where[c]], bit=fchars.index(c) if c in fchars else -1,
+                       sigma=c in sig_chars)
+            virt_of[("b", c)] = v
+        elif item[0] == "F":
+            grp = layout["groups"][item[1]]
+            v = P.add_virt(_prod(dims[c] for c in grp), enc=True)
+            for c in grp:
+                tg = [(i, 1) for i in where[c]]
+                if len(tg) > 2:
+                    raise NotImplementedError("index '%s' repeated more than twice on one operand" % c)
+                P.add_comp(v, dims[c], tg, bit=fchars.index(c) if c in fchars else -1, sigma=c in sig_chars)
+            virt_of[("F", item[1])] = v
+    blocked = layout["blocked"]
+    for item in layout["axes"]:
+        if item[0] == "b":
+            P.add_out(dims[item[1]], [(virt_of[("b", item[1])], 1)])
+        elif item[0] == "P":
+            P.add_out(2, [(virt_of[("F", g)], 1) for g in layout["pgroups"]])
+        else:
+            full = _prod(dims[c] for c in layout["groups"][item[1]])
+            if blocked:
+                P.add_out(full // 2, [(virt_of[("F", item[1])], 2)])
+            else:
+                P.add_out(full, [(virt_of[("F", item[1])], 1)])
+    P.set_table(codes)
+    return P.run(T.data)
+
+
+def _einsum2(plan, A, B, first, ignore):
+    sA, sB = plan.inputs
+    out = plan.output
+    dims = plan.dims
+    setA, setB, setO = set(sA), set(sB), set(out)
+    uA, uB = list(dict.fromkeys(sA)), list(dict.fromkeys(sB))
+    if A.data.is_complex() or B.data.is_complex():
+        raise NotImplementedError("complex contractions are not provided by this build")
+    if A.data.dtype != B.data.dtype:
+        raise NotImplementedError("operands of different dtypes")
+    isF = lambda c: _fermionic(plan, c)
+    batch = [c for c in uA if c in setB and c in setO]
+    contr = [c for c in uA if c in setB and c not in setO]
+    freeA = [c for c in out if c in setA and c not in setB]
+    freeB = [c for c in out if c in setB and c not in setA]
+    redA = [c for c in uA if c not in setB and c not in setO]
+    redB = [c for c in uB if c not in setA and c not in setO]
+    RB, RF = [c for c in freeA if not isF(c)], [c for c in freeA if isF(c)]
+    KB, KF = [c for c in contr if not isF(c)], [c for c in contr if isF(c)]
+    NB, NF = [c for c in freeB if not isF(c)], [c for c in freeB if isF(c)]
+    for c in batch:
+        if isF(c):
+            fail("Error[einsum]: Inconsistent index statistics.")
+    # a fermionic group of total dimension 1 carries no parity
+    hasR = _prod(dims[c] for c in RF) >= 2
+    hasK = _prod(dims[c] for c in KF) >= 2
+    hasN = _prod(dims[c] for c in NF) >= 2
+    blocked = A.is_even() and B.is_even()
+    split = blocked and hasR and hasK and hasN
+    fa = [c for c in uA if isF(c)]
+    fb = [c for c in uB if isF(c)]
+    fo = [c for c in out if isF(c)]
+    if ignore:
+        partA = partB = partO = ({}, {})
+    else:
+        partA, partB, partO = plan.split_form([set(fa), set(fb), set(fo)])
+    sigA = set() if ignore else {c for c in plan.pairs if c in setA}
+    sigB = set() if ignore else {c for c in plan.pairs if c not in setA}
+    codesA = sign_table(fa, *partA)
+    codesB = sign_table(fb, *partB)
+    codesO = sign_table(fo, *partO, zero_if_odd=blocked)
+
+    def layout_for(rowsB, rowsF, hasRows, colsB, colsF, hasCols, red, gname):
+        axes = [("b", c) for c in batch]
+        if split:
+            axes.append(("P",))
+        axes += [("b", c) for c in rowsB]
+        if hasRows:
+            axes.append(("F", gname[0]))
+        axes += [("b", c) for c in colsB]
+        if hasCols:
+            axes.append(("F", gname[1]))
+        axes += [("b", c) for c in red]
+        groups = {gname[0]: rowsF, gname[1]: colsF}
+        return {"axes": axes, "groups": groups, "blocked": blocked, "pgroups": list(gname)}
+
+    # fermionic groups of dimension 1 are gathered as plain legs
+    def plainify(Fs, has):
+        return ([], Fs) if has else (Fs, [])
+
+    RFb, RFf = plainify(RF, hasR)
+    KFb, KFf = plainify(KF, hasK)
+    NFb, NFf = plainify(NF, hasN)
+    layA = layout_for(RB + RFb, RFf, hasR, KB + KFb, KFf, hasK, redA, ("R", "K"))
+    layB = layout_for(KB + KFb, KFf, hasK, NB + NFb, NFf, hasN, redB, ("K", "N"))
+    Ag = _operand_gather(A, sA, plan, layA, fa, codesA, sigA)
+    Bg = _operand_gather(B, sB, plan, layB, fb, codesB, sigB)
+    half = 2 if blocked else 1
+    nb = _prod(dims[c] for c in batch)
+    nP = 2 if split else 1
+    M = _prod(dims[c] for c in RB + RFb) * (_prod(dims[c] for c in RFf) // half if hasR else 1)
+    K = _prod(dims[c] for c in KB + KFb) * (_prod(dims[c] for c in KFf) // half if hasK else 1)
+    N = _prod(dims[c] for c in NB + NFb) * (_prod(dims[c] for c in NFf) // half if hasN else 1)
+    ra, rb = _prod(dims[c] for c in redA), _prod(dims[c] for c in redB)
+    lib = L.lib()
+    if ra > 1:
+        t = DeviceArray.empty(Ag.numel() // ra, Ag.dtype)
+        L.check(lib.gtn_reduce_last(L.ptr(Ag), L.ptr(t), t.numel(), ra, L.dtype_code(Ag), L.stream_ptr()), "reduce")
+        Ag = t
+    if rb > 1:
+        t = DeviceArray.empty(Bg.numel() // rb, Bg.dtype)
+        L.check(lib.gtn_reduce_last(L.ptr(Bg), L.ptr(t), t.numel(), rb, L.dtype_code(Bg), L.stream_ptr()), "reduce")
+        Bg = t
+    Cg = _contract_blocks(Ag, Bg, nb, nP, M, N, K)
+    if not plan.has_output:
+        return Cg.numpy().reshape(-1)[0].item()
+    # output gather from the GEMM layout [batch][P][RB][RFh][NB][NFw]
+    shapeC = [dims[c] for c in batch] + ([2] if split else []) + [dims[c] for c in RB + RFb]
+    if hasR:
+        shapeC.append(_prod(dims[c] for c in RFf) // half)
+    shapeC += [dims[c] for c in NB + NFb]
+    if hasN:
+        shapeC.append(_prod(dims[c] for c in NFf) // half)
+    stC = strides(shapeC)
+    P = Perm()
+    pos = 0
+    src = {}
+    for c in batch:
+        src[c] = (P.add_in(dims[c], stC[pos]), 1)
+        pos += 1
+    p_stride = 0
+    if split:
+        p_stride = stC[pos]
+        pos += 1
+    for c in RB + RFb:
+        src[c] = (P.add_in(dims[c], stC[pos]), 1)
+        pos += 1
+    if hasR:
+        full = _prod(dims[c] for c in RFf)
+        i = P.add_in(full, stC[pos], enc=True, split=blocked, stride_lo=p_stride)
+        pos += 1
+        m = full
+        for c in RFf:
+            m //= dims[c]
+            src[c] = (i, m)
+    for c in NB + NFb:
+        src[c] = (P.add_in(dims[c], stC[pos]), 1)
+        pos += 1
+    if hasN:
+        full = _prod(dims[c] for c in NFf)
+        i = P.add_in(full, stC[pos], enc=True, split=blocked, stride_lo=0)
+        pos += 1
+        m = full
+        for c in NFf:
+            m //= dims[c]
+            src[c] = (i, m)
+    for c in out:
+        st = plan.stat_of[c][0]
+        v = P.add_virt(dims[c], enc=_out_encoder_pp(first, st), sigma=(first.format == "matrix" and st == -1))
+        i, m = src[c]
+        P.add_comp(v, dims[c], [(i, m)], bit=fo.index(c) if c in fo else -1)
+        P.add_out(dims[c], [(v, 1)])
+    P.set_table(codesO)
+    res = dense.__new__(dense)
+    res.data = P.run(Cg)
+    res.statistics = plan.out_stats
+    res.encoder, res.format = first.encoder, first.format
+    res._even = True if blocked else None
+    return res
+
+
+def _contract_blocks(Ag, Bg, nb, nP, M, N, K):
+    """C[b][p] = A[b][p] (M x K) * B[b][p] (K x N) for every batch b and parity block p."""
+    dt = L.dtype_code(Ag)
+    prec = settings.precision if Ag.dtype == np.float64 else "f32"
+    Cg = DeviceArray.empty(nb * nP * M * N, Ag.dtype)
+    small = M * N * nb * nP <= 256 and K >= 8192
+    if small:
+        # split K so that a dot-product-shaped contraction spreads over many workgroups
+        S = 1
+        while S < 256 and K % (2 * S) == 0 and K // (2 * S) >= 512:
+            S *= 2
+        part = DeviceArray.empty(nb * nP * M * N * S, Ag.dtype)
+        groups = []
+        for b in range(nb):
+            for p in range(nP):
+                base = (b * nP + p)
+                es = Ag.element_size()
+                groups.append(_gemm_desc(Ag.data_ptr() + es * base * M * K, K, 1, Bg.data_ptr() + es * base * K * N, N, 1,
+                                         part.data_ptr() + es * base * M * N * S, N * S, S, M, N, K // S, batch=S,
+                                         sab=K // S, sbb=(K // S) * N, scb=1))
+        _gemm(groups, dt, prec)
+        L.check(L.lib().gtn_reduce_last(L.ptr(part), L.ptr(Cg), nb * nP * M * N, S, dt, L.stream_ptr()), "reduce")
+        return Cg
+    esz = Ag.element_size()
+    groups = []
+    for p in range(nP):
+        groups.append(_gemm_desc(Ag.data_ptr() + esz * p * M * K, K, 1, Bg.data_ptr() + esz * p * K * N, N, 1,
+                                 Cg.data_ptr() + esz * p * M * N, N, 1, M, N, K, batch=nb,
+                                 sab=nP * M * K, sbb=nP * K * N, scb=nP * M * N))
+    _gemm(groups, dt, prec)
+    return Cg
+
+
+# --------------------------------------------------------------------------------------------------
+# join_legs / split_legs (__init__.py:2947-3316)
+# --------------------------------------------------------------------------------------------------
+def _grouping_sign_axes(srt, inter):
+    """positions (in the original leg order) that get (-1)^p: +1 legs in a group of intermediate -1"""
+    if len(srt) != len(inter):
+        fail("Error[get_grouping_sign_factors]: Inconsistent number of intermediate_stat and groupings!")
+    out = set()
+    for g, (chars, st, dm, pos) in enumerate(srt):
+        for t, p in zip(st, pos):
+            if t == 1 and inter[g] == -1:
+                out.add(p)
+    return out
+
+
+def _intermediate(srt, inter):
+    """Per group: (boson dim, fermion dim, stats of the fused legs, final stat)."""
+    res = []
+    for g, (chars, st, dm, pos) in enumerate(srt):
+        bd = _prod(d for t, d in zip(st, dm) if t in BOSE)
+        fd = _prod(d for t, d in zip(st, dm) if t in FERMI)
+        bn = sum(1 for t in st if t in BOSE)
+        fn = sum(1 for t in st if t in FERMI)
+        final = HYBRID if (bn and fn) else inter[g]
+        res.append((bd, fd, bn, fn, final))
+    return res
+
+
+def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, save_memory=False):
+    """Fuse groups of legs into one leg each; the result is parity-preserving (__init__.py:2947-3068).
+    Bosons move left inside a group, +1 legs in a -1 group pick up (-1)^p, the fermionic legs fuse in
+    C order (first leg = high bits), matrix format applies sigma to each fused -1 leg, then the
+    encoder switches on the fused leg."""
+    string_inp = denumerate(string_inp.replace(" ", ""))
+    inter = tuple(intermediate_stat) if not np.isscalar(intermediate_stat) else (intermediate_stat,)
+    groups, srt = group_info(string_inp, T.statistics, T.shape)
+    sgn_pos = _grouping_sign_axes(srt, inter)
+    info = _intermediate(srt, inter)
+    P = Perm()
+    st = strides(T.shape)
+    std = T.format == "matrix"  # input must first go to standard: sigma on its -1 legs
+    ins = [P.add_in(T.shape[a], st[a], enc=_is_pp(T, a), sigma=(std and T.statistics[a] == -1)) for a in range(T.ndim)]
+    nb = len(sgn_pos)
+    bitpos = {p: k for k, p in enumerate(sorted(sgn_pos))}
+    fin_shape, fin_stats = [], []
+    for g, (chars, stg, dm, pos) in enumerate(srt):
+        bd, fd, bn, fn, final = info[g]
+        # the fused leg: bosons (high) then fermions; the fermionic part is switched to the
+        # parity-preserving encoder over its own dimension, matrix format applies sigma to it
+        if bn and fn:
+            vb = P.add_virt(bd)
+            for t, d, p in zip(stg, dm, pos):
+                if t in BOSE:
+                    P.add_comp(vb, d, [(ins[p], 1)])
+            vf = P.add_virt(fd, enc=True, sigma=(make_format == "matrix" and inter[g] == -1))
+            for t, d, p in zip(stg, dm, pos):
+                if t in FERMI:
+                    P.add_comp(vf, d, [(ins[p], 1)], bit=bitpos.get(p, -1))
+            # two raw axes (boson high, fermion low) whose C-order product is the fused leg
+            P.add_out(bd, [(vb, 1)])
+            P.add_out(fd, [(vf, 1)])
+        elif fn:
+            vf = P.add_virt(fd, enc=inter[g] in FERMI, sigma=(make_format == "matrix" and inter[g] == -1))
+            for t, d, p in zip(stg, dm, pos):
+                P.add_comp(vf, d, [(ins[p], 1)], bit=bitpos.get(p, -1))
+            P.add_out(fd, [(vf, 1)])
+        else:
+            # a purely bosonic group is neither encoded nor sigma-scaled (its intermediate leg has
+            # statistic 0) but ends up labelled with the requested intermediate statistic
+            vb = P.add_virt(bd)
+            for t, d, p in zip(stg, dm, pos):
+                P.add_comp(vb, d, [(ins[p], 1)])
+            P.add_out(bd, [(vb, 1)])
+        fin_shape.append(bd * fd)
+        fin_stats.append(final)
+    if nb:
+        P.set_table([bin(m).count("1") & 1 for m in range(1 << nb)])
+    out = T._like(P.run(T.data), statistics=fin_stats, encoder="parity-preserving", format=make_format,
+                  even=T._even)
+    out.data = out.data.reshape(fin_shape)
+    return out
+
+
+def split_legs(T, string_inp, final_stat, final_shape, intermediate_stat=None, save_memory=False):
+    """Inverse of join_legs (__init__.py:3070-3170), including its early return for matrix-format
+    input: such input comes back canonical and in matrix format."""
+    string_inp = denumerate(string_inp.replace(" ", ""))
+    inter = tuple(intermediate_stat) if not np.isscalar(intermediate_stat) else (intermediate_stat,)
+    final_stat = tuple(final_stat)
+    final_shape = tuple(final_shape)
+    groups, srt = group_info(string_inp, final_stat, final_shape)
+    sgn_pos = _grouping_sign_axes(srt, inter)
+    info = _intermediate(srt, inter)
+    fmt, enc = T.format, T.encoder
+    matrix = fmt == "matrix"
+    out_enc_pp = (not matrix) and enc == "parity-preserving"
+    P = Perm()
+    st = strides(T.shape)
+    # source: fused legs of T; for a hybrid leg the bosonic part is the high factor
+    src_of = {}
+    for g, (chars, stg, dm, pos) in enumerate(srt):
+        bd, fd, bn, fn, final = info[g]
+        fused_pp = enc == "parity-preserving"
+        if bn and fn:
+            ib = P.add_in(bd, st[g] * fd)
+            i_f = P.add_in(fd, st[g], enc=fused_pp, sigma=(matrix and inter[g] == -1))
+            mb, mf = bd, fd
+            for c, t, d, p in zip(chars, stg, dm, pos):
+                if t in BOSE:
+                    mb //= d
+                    src_of[p] = (ib, mb)
+                else:
+                    mf //= d
+                    src_of[p] = (i_f, mf)
+        else:
+            full = bd * fd
+            i_ = P.add_in(full, st[g], enc=fused_pp and fn > 0 and inter[g] in FERMI,
+                          sigma=(matrix and fn > 0 and inter[g] == -1))
+            m = full
+            for c, t, d, p in zip(chars, stg, dm, pos):
+                m //= d
+                src_of[p] = (i_, m)
+    bitpos = {p: k for k, p in enumerate(sorted(sgn_pos))}
+    for p, (t, d) in enumerate(zip(final_stat, final_shape)):
+        # output leg p: encoder of the result, matrix format re-applied on final -1 legs
+        v = P.add_virt(d, enc=out_enc_pp and t in FERMI, sigma=(matrix and t == -1))
+        i_, m = src_of[p]
+        P.add_comp(v, d, [(i_, m)], bit=bitpos.get(p, -1))
+        P.add_out(d, [(v, 1)])
+    if sgn_pos:
+        P.set_table([bin(m).count("1") & 1 for m in range(1 << len(sgn_pos))])
+    data = P.run(T.data)
+    res = T._like(data, statistics=final_stat, encoder=("canonical" if matrix else enc), format=fmt, even=T._even)
+    return res
+
+
+# --------------------------------------------------------------------------------------------------
+# svd / eig (__init__.py:3931-4698)
+# --------------------------------------------------------------------------------------------------
+def _block_decompose_device(Mdata, cutoff, mode):
+    """Parity blocks of a parity-preserving (-1,+1) matrix, decomposed on the device.
+    Returns (U, S, Vt) full interleaved matrices exactly as BlockSVD / BlockEig lay them out
+    (__init__.py:3953-4029, 4347-4422): int(cutoff/2) per block, padded to a power of two."""
+    m, n = Mdata.shape
+    if not _pkg_flag("skip_parity_blocking_check", False):
+        leak = _parity_leak(Mdata)
+        if leak / (m * n / 2) > 1.0e-14:
+            fail("Error[BlockSVD]: This matrix is not constructed from a Grassmann-even tensor.")
+    ME = _strided_block(Mdata, 0)
+    MO = _strided_block(Mdata, 1)
+    half = None if cutoff is None else int(cutoff / 2)
+    blocks = []
+    for Mb in (ME, MO):
+        mb, nb_ = Mb.shape
+        k = min(mb, nb_)
+        if half is not None:
+            k = min(k, half)
+        blocks.append((Mb, k))
+    outs = _svd_blocks_raw(blocks, mode)
+    # rank rule: count nonzero singular values per block (k values read back)
+    kk = [int(np.count_nonzero(o[1].numpy())) if o[3] > 0 else 0 for o in outs]
+    d = max(kk)
+    d = int(2 ** math.ceil(math.log2(d))) if d > 0 else 1
+    mh, nh = m // 2, n // 2
+    dt = Mdata.dtype
+    # stack the kept columns / rows of both blocks, zero-padded to d, then interleave by parity
+    SU = DeviceArray.zeros((2, mh, d), dt)
+    SV = DeviceArray.zeros((2, d, nh), dt)
+    sv = DeviceArray.zeros((2 * d,), dt)
+    for par, (U_b, S_b, V_b, k) in enumerate(outs):
+        kp = kk[par]
+        if kp == 0:
+            continue
+        copy2d(SU, d, U_b, k, mh, kp, dst_off=par * mh * d)
+        copy2d(SV, nh, V_b, n // 2, kp, nh, dst_off=par * d * nh)
+        copy2d(sv, 2, S_b, 1, kp, 1, dst_off=par)  # sv[2j + par] = s_par[j]
+    U = _interleave(SU, m, 2 * d, row_stride=d, par_stride=mh * d)
+    Vt = _interleave(SV, 2 * d, n, row_stride=nh, par_stride=d * nh)
+    S = DeviceArray.zeros((2 * d, 2 * d), dt)
+    copy2d(S, 2 * d + 1, sv, 1, 2 * d, 1)  # the diagonal
+    return U, S, Vt
+
+
+def _interleave(stack, rows, cols, row_stride, par_stride):
+    """full[r][c] = stack[r % 2][r // 2][c // 2] when r and c have equal parity, else 0 (both dims
+    powers of two): the interleaving of get_full_matrix (__init__.py:4017-4028)."""
+    P = Perm()
+    ir = P.add_in(rows, row_stride, enc=True, split=True, stride_lo=par_stride)
+    ic = P.add_in(cols, 1, enc=True, split=True, stride_lo=0)
+    vr = P.add_virt(rows, enc=True)
+    P.add_comp(vr, rows, [(ir, 1)], bit=0)
+    vc = P.add_virt(cols, enc=True)
+    P.add_comp(vc, cols, [(ic, 1)], bit=1)
+    P.add_out(rows, [(vr, 1)])
+    P.add_out(cols, [(vc, 1)])
+    P.set_table([0, 2, 2, 0])  # zero unless both indices have the same parity
+    return P.run(stack)
+
+
+def _parity_leak(M):
+    """Frobenius norm of the parity-odd entries M[i, j], (i + j) odd, of a parity-preserving matrix.
+    (The reference sums |M_ij|, __init__.py:3977-3984; both are zero exactly for an even tensor.)"""
+    lib = L.lib()
+    dims = (C.c_int64 * 2)(*M.shape)
+    one = (C.c_int32 * 2)(1, 1)
+    out = DeviceArray.empty(1, np.float64)
+    L.check(lib.gtn_odd_norm(L.ptr(M), 2, dims, one, one, L.dtype_code(M), L.ptr(out), L.stream_ptr()), "gtn_odd_norm")
+    return math.sqrt(float(out.item()))
+
+
+def _strided_block(M, par):
+    """Contiguous copy of M[par::2, par::2] (the ME / MO extraction of __init__.py:3990-3996)."""
+    m, n = M.shape
+    P = Perm()
+    i0 = P.add_in(max(m - 1, 1), n)
+    i1 = P.add_in(max(n - 1, 1), 1)
+    v0 = P.add_virt(m // 2)
+    P.add_comp(v0, m // 2, [(i0, 2)])
+    v1 = P.add_virt(n // 2)
+    P.add_comp(v1, n // 2, [(i1, 2)])
+    P.add_out(m // 2, [(v0, 1)])
+    P.add_out(n // 2, [(v1, 1)])
+    off = par * n + par
+    return P.run(M.flat(off, M.numel() - off))
+
+
+def _svd_blocks_raw(blocks, mode):
+    """blocks: list of (contiguous FP64 matrix, k).  Returns per block (U (m x k), S (k), Vt (k x n), k)."""
+    lib = L.lib()
+    nblk = len(blocks)
+    arr = (L.SvdBlock * nblk)()
+    outs = []
+    for i, (Mb, k) in enumerate(blocks):
+        mb, nb_ = Mb.shape
+        kk = k if (mb and nb_) else 0
+        U = DeviceArray.zeros((mb, max(kk, 1)), np.float64)
+        S = DeviceArray.zeros((max(kk, 1),), np.float64)
+        V = DeviceArray.zeros((max(kk, 1), nb_), np.float64)
+        outs.append((U, S, V, kk))
+        arr[i].M = Mb.data_ptr() if Mb.numel() else None
+        arr[i].U, arr[i].S, arr[i].Vt = U.data_ptr(), S.data_ptr(), V.data_ptr()
+        arr[i].m, arr[i].n, arr[i].lda, arr[i].ldu, arr[i].ldv = mb, nb_, max(nb_, 1), max(kk, 1), max(nb_, 1)
+        arr[i].k = kk
+    opts = L.SvdOpts()
+    opts.oversample = settings.svd_oversample
+    opts.iters = settings.svd_iters
+    opts.precision = L.PREC[settings.svd_precision]
+    opts.mode = mode
+    opts.seed = settings.seed
+    ws_bytes = lib.gtn_svd_workspace_bytes(arr, nblk, C.byref(opts))
+    ws = DeviceArray.empty(max(ws_bytes // 8, 1), np.float64)
+    L.check(lib.gtn_svd_blocks(arr, nblk, C.byref(opts), L.ptr(ws), ws_bytes, L.stream_ptr()), "gtn_svd_blocks")
+    return outs
+
+
+def _decompose(T, string, cutoff, mode):
+    name = "svd" if mode == 0 else "eig"
+    if not isinstance(T, dense):
+        fail("Error[%s]: Object type must only be dense or sparse!" % name)
+    if T.data.is_complex() or T.data.dtype != np.float64:
+        raise NotImplementedError("%s of non-FP64-real tensors is not provided by this build" % name)
+    for st in T.statistics:
+        if st not in ALLOWED_STAT:
+            fail("Error[%s]: The input object contains illegal statistics. (0, 1, -1, or * only)" % name)
+    left, right = parse_partition(string, name)
+    fmt, enc = T.format, T.encoder
+    xstats, xshape = T.statistics, T.shape
+    nl = len(left)
+    sl, sr = xstats[:nl], xstats[nl:]
+    shl, shr = xshape[:nl], xshape[nl:]
+    lb1 = list(sl) in ([-1], [1]) and list(shl) == [1]
+    rb1 = list(sr) in ([-1], [1]) and list(shr) == [1]
+    if lb1 or rb1:
+        # a single fermionic leg of dimension one (__init__.py:4132-4170)
+        nrm = T.norm
+        M = T.force_format("matrix")
+        if lb1:
+            U = dense([[1.0]], statistics=(*sl, 1), format="matrix")
+            S = dense([[nrm]], statistics=(-1, 1), format="matrix")
+            V = M / nrm
+            V.statistics = (-1,) + tuple(V.statistics[1:])
+        else:
+            U = M / nrm
+            S = dense([[nrm]], statistics=(-1, 1), format="matrix")
+            V = dense([[1.0]], statistics=(-1, *sr), format="matrix")
+            U.statistics = tuple(U.statistics[:-1]) + (1,)
+        out = [U, S, V]
+        if fmt == "standard":
+            out = [x.switch_format() for x in out]
+        if enc == "parity-preserving":
+            out = [x.switch_encoder() for x in out]
+        return tuple(out)
+    inter = (0 if all(s == 0 for s in sl) else -1, 0 if all(s == 0 for s in sr) else 1)
+    M = join_legs(T, "(" + left + ")(" + right + ")", "matrix", intermediate_stat=inter)
+    M = M.force_encoder("parity-preserving")
+    Md = M.data
+    if M.statistics[0] == 0 or M.statistics[1] == 0:
+        mm, nn = Md.shape
+        k = min(mm, nn) if cutoff is None else min(mm, nn, cutoff)
+        U_b, S_b, V_b, k = _svd_blocks_raw([(Md, k)], mode)[0]
+        nz = int(np.count_nonzero(S_b.numpy())) if k else 0
+        U = DeviceArray.zeros((mm, max(nz, 1)), np.float64)
+        Vt = DeviceArray.zeros((max(nz, 1), nn), np.float64)
+        S = DeviceArray.zeros((max(nz, 1), max(nz, 1)), np.float64)
+        if nz:
+            copy2d(U, nz, U_b, k, mm, nz)
+            copy2d(Vt, nn, V_b, nn, nz, nn)
+            copy2d(S, nz + 1, S_b, 1, nz, 1)
+    else:
+        U, S, Vt = _block_decompose_device(Md, cutoff, mode)
+    ls, rs = -1, 1
+    s0, s1 = M.statistics
+
+    def mk(data, stats):
+        r = dense.__new__(dense)
+        r.data, r.statistics, r.encoder, r.format, r._even = data, tuple(stats), "parity-preserving", "matrix", True
+        return r
+
+    if s0 == 0:
+        U, S, V = mk(U, (0, 0)), mk(S, (0, 0)), mk(Vt, (0, s1))
+        ls = rs = 0
+    elif s1 == 0:
+        U, S, V = mk(U, (s0, 0)), mk(S, (0, 0)), mk(Vt, (0, 0))
+        ls = rs = 0
+    else:
+        U, S, V = mk(U, (s0, 1)), mk(S, (-1, 1)), mk(Vt, (-1, s1))
+    dl = S.shape[0]
+    ustats = tuple(xstats[:nl]) + (-ls,)
+    vstats = (-rs,) + tuple(xstats[nl:])
+    ushape = tuple(xshape[:nl]) + (dl,)
+    vshape = (dl,) + tuple(xshape[nl:])
+    nc = fresh_char(left + right)
+    U = split_legs(U, "(" + left + ")" + nc, ustats, ushape, intermediate_stat=(-1, 1))
+    S = S.switch_encoder()
+    V = split_legs(V, nc + "(" + right + ")", vstats, vshape, intermediate_stat=(-1, 1))
+    out = [U, S, V]
+    if fmt == "standard":
+        out = [x.switch_format() for x in out]
+    if enc == "parity-preserving":
+        out = [x.switch_encoder() for x in out]
+    for x in out:
+        x._even = True
+    return tuple(out)
+
+
+def svd(InpObj, string, cutoff=None, save_memory=False):
+    """(__init__.py:4033-4308) U, S, V with T = einsum(U S V); parity blocks kept int(cutoff/2) each."""
+    return _decompose(InpObj, string, cutoff, 0)
+
+
+def eig(InpObj, string, cutoff=None, debug_mode=False, save_memory=False):
+    """(__init__.py:4425-4698) U, Lambda, U^+ of a Hermitian tensor; eigenvalues ordered by |lambda|."""
+    M = InpObj
+    return _decompose(M, string, cutoff, 1)
+
+
+# --------------------------------------------------------------------------------------------------
+# hconjugate (__init__.py:5300-5493)
+# --------------------------------------------------------------------------------------------------
+def hconjugate(InpObj, string, save_memory=False):
+    from .block import block, hconjugate_block
+    if isinstance(InpObj, block):
+        return hconjugate_block(InpObj, string)
+    T = InpObj
+    if T.data.is_complex():
+        raise NotImplementedError("complex hconjugate is not provided by this build")
+    left, right = parse_partition(string, "hconjugate")
+    fmt, enc = T.format, T.encoder
+    xstats, xshape = T.statistics, T.shape
+    nl = len(left)
+    M = join_legs(T, "(" + left + ")(" + right + ")", "matrix", intermediate_stat=(-1, 1))
+    M = M.force_encoder("parity-preserving")
+    # conjugate transpose (real data: transpose) as a gather
+    Mt = _relayout(M, [1, 0], out_encoder=M.encoder)
+    s0, s1 = M.statistics
+    M = M._like(Mt, statistics=((-s1 if s1 in FERMI else s1), (-s0 if s0 in FERMI else s0)))
+    new_stats = list(xstats[nl:]) + list(xstats[:nl])
+    new_stats = tuple(-s if s in FERMI else s for s in new_stats)
+    new_shape = tuple(xshape[nl:]) + tuple(xshape[:nl])
+    O = split_legs(M, "(" + right + ")(" + left + ")", new_stats, new_shape, intermediate_stat=(-1, 1))
+    if fmt != O.format:
+        O = O.switch_format()
+    if enc != O.encoder:
+        O = O.switch_encoder()
+    O._even = T._even
+    return O
+
+
+# --------------------------------------------------------------------------------------------------
+# power / sqrt (__init__.py:6050-6069)
+# --------------------------------------------------------------------------------------------------
+def power(T, p, rcond=1e-10):
+    from .block import block, power_block
+    if isinstance(T, block):
+        return power_block(T, p)
+    fmt, enc = T.format, T.encoder
+    M = T.force_format("matrix").force_encoder("canonical")
+    lib = L.lib()
+    L.check(lib.gtn_power_diag(L.ptr(M.data), M.data.numel(), 1, L.dtype_code(M.data), float(p), float(rcond),
+                               L.stream_ptr()), "gtn_power_diag")
+    return M.force_format(fmt).force_encoder(enc)
+
+
+def sqrt(T, rcond=1e-10):
+    return power(T, 0.5)

@@ -1,0 +1,397 @@
+// Grouped, batched MFMA GEMM for the parity-block contractions (replaces oe.contract at
+// __init__.py:2295 and einsum_block's per-sector np.einsum at :2781).
+//
+// One launch covers up to kMaxGroups independent GEMMs (the nonzero parity sectors of a
+// contraction, or the parity blocks of a decomposition).  Workgroups of 256 lanes (4 wave64s in a
+// 2x2 arrangement) compute a BM x BN tile; operand tiles are staged through LDS.  Precision is a
+// template parameter:
+//   F64    : v_mfma_f64_16x16x4f64, FP64 accumulation
+//   F32    : operands rounded to FP32 on their way into LDS, v_mfma_f32_16x16x4f32
+//   BF16Xn : operands split into n BF16 terms on their way into LDS (hi/lo for n=2, hi/mid/lo for
+//            n=3), v_mfma_f32_16x16x32_bf16 on the 3 (n=2) or 6 (n=3) leading cross products,
+//            FP32 accumulation.
+// The element type in HBM stays FP64 (or FP32); the reduced precision lives only between LDS and
+// the matrix cores.
+#include "gtn_common.h"
+#include <string.h>
+
+namespace gtn {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMaxGroups = 12;
+
+struct GemmGroup {
+    const void *A;
+    const void *B;
+    void *C;
+    int32_t M, N, K, batch;
+    int32_t tiles_m, tiles_n;
+    int64_t sam, sak, sbk, sbn, scm, scn, sab, sbb, scb;
+    double alpha, beta;
+};
+
+struct GemmArgs {
+    int32_t n_groups;
+    int32_t tile_start[kMaxGroups + 1];  // prefix sum of tiles (incl. batch) per group
+    GemmGroup g[kMaxGroups];
+};
+
+// Locate this workgroup's (group, batch, tile_m, tile_n).  Tiles are enumerated per XCD in
+// contiguous ranges so that workgroups sharing operand panels share an L2.
+__device__ __forceinline__ bool locate(const GemmArgs &a, int &gi, int &b, int &tm, int &tn) {
+    int total = a.tile_start[a.n_groups];
+    int bid = blockIdx.x;
+    int per = (total + 7) / 8;
+    int xcd = bid % 8, slot = bid / 8;
+    int t = xcd * per + slot;
+    // blocks beyond `total` are launched only to round the grid to a multiple of 8
+    if (t >= total) return false;
+    gi = 0;
+    while (gi + 1 < a.n_groups && t >= a.tile_start[gi + 1]) ++gi;
+    t -= a.tile_start[gi];
+    const GemmGroup &g = a.g[gi];
+    int per_b = g.tiles_m * g.tiles_n;
+    b = t / per_b;
+    t -= b * per_b;
+    tm = t / g.tiles_n;
+    tn = t - tm * g.tiles_n;
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ double ld(const T *p, int64_t i) { return (double)p[i]; }
+
+// Load a (rows x BK) panel of an operand whose element (r, k) is at p[r*sr + k*sk] into
+// registers, one element per slot; out-of-range elements read as zero.  `kc` selects the mapping
+// of lanes to elements so that consecutive lanes read consecutive addresses.
+template <typename T, int ROWS, int BK>
+__device__ __forceinline__ void load_panel(const T *p, int64_t sr, int64_t sk, int r0, int k0, int R, int K,
+                                           double (&reg)[ROWS * BK / 256]) {
+    constexpr int N = ROWS * BK / 256;
+    const bool kc = (sk == 1);
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+        int e = threadIdx.x + 256 * s;
+        int r, k;
+        if (kc) {
+            r = e / BK;
+            k = e % BK;
+        } else {
+            k = e / ROWS;
+            r = e % ROWS;
+        }
+        int gr = r0 + r, gk = k0 + k;
+        reg[s] = (gr < R && gk < K) ? ld(p, gr * sr + gk * sk) : 0.0;
+    }
+}
+template <int ROWS, int BK>
+__device__ __forceinline__ void panel_coord(int s, bool kc, int &r, int &k) {
+    int e = threadIdx.x + 256 * s;
+    if (kc) {
+        r = e / BK;
+        k = e % BK;
+    } else {
+        k = e / ROWS;
+        r = e % ROWS;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 / FP32 kernels: 16x16x4 MFMA, LDS panels stored k-major [BK][ROWS + pad].
+// MFMA 16x16x4 operand layout (per lane l of the wave): A value = A[i = l%16][k = l/16],
+// B value = B[k = l/16][j = l%16]; result lane l holds C[4*(l/16) + v][l%16], v = 0..3.
+// ------------------------------------------------------------------------------------------------
+template <typename TIN, typename TC, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
+    int gi, b, tm, tn;
+    if (!locate(args, gi, b, tm, tn)) return;
+    const GemmGroup &g = args.g[gi];
+    const TIN *A = (const TIN *)g.A + b * g.sab;
+    const TIN *B = (const TIN *)g.B + b * g.sbb;
+    TIN *C = (TIN *)g.C + b * g.scb;
+    constexpr int PAD = 2;
+    __shared__ TC As[BK][BM + PAD];
+    __shared__ TC Bs[BK][BN + PAD];
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    constexpr int WM = BM / 2, WN = BN / 2, TI = WM / 16, TJ = WN / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    const int m0 = tm * BM, n0 = tn * BN;
+    using acc_t = typename std::conditional<sizeof(TC) == 8, d4, f4>::type;
+    acc_t acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    double ra[NA], rb[NB];
+    const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
+    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, g.K, ra);
+    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, g.K, rb);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NA; ++s) {
+            int r, k;
+            panel_coord<BM, BK>(s, akc, r, k);
+            As[k][r] = (TC)ra[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NB; ++s) {
+            int r, k;
+            panel_coord<BN, BK>(s, bkc, r, k);
+            Bs[k][r] = (TC)rb[s];
+        }
+        __syncthreads();
+        if (k0 + BK < g.K) {  // prefetch the next panels while the matrix cores work
+            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, g.K, ra);
+            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, g.K, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            TC af[TI], bf[TJ];
+#pragma unroll
+            for (int i = 0; i < TI; ++i) af[i] = As[kk + lane / 16][wm + 16 * i + lane % 16];
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) bf[j] = Bs[kk + lane / 16][wn + 16 * j + lane % 16];
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    if constexpr (sizeof(TC) == 8)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    else
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+    const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = m0 + wm + 16 * i + 4 * (lane / 16) + v;
+                int c = n0 + wn + 16 * j + lane % 16;
+                if (r < g.M && c < g.N) {
+                    int64_t o = r * g.scm + c * g.scn;
+                    double val = alpha * (double)acc[i][j][v];
+                    if (beta != 0.0) val += beta * (double)C[o];
+                    C[o] = (TIN)val;
+                }
+            }
+}
+
+// ------------------------------------------------------------------------------------------------
+// BF16 split-product kernel: 16x16x32 MFMA.  LDS panels stored row-major [ROWS][BK + pad] so that a
+// lane's 8 consecutive k values are one 16-byte read.  Operand layout (lane l): A values
+// A[i = l%16][k = 8*(l/16) + t], t = 0..7; B values B[k = 8*(l/16) + t][j = l%16]; the result layout
+// is that of the 16x16x4 instructions.
+// ------------------------------------------------------------------------------------------------
+template <typename TIN, int NSPLIT, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
+    int gi, b, tm, tn;
+    if (!locate(args, gi, b, tm, tn)) return;
+    const GemmGroup &g = args.g[gi];
+    const TIN *A = (const TIN *)g.A + b * g.sab;
+    const TIN *B = (const TIN *)g.B + b * g.sbb;
+    TIN *C = (TIN *)g.C + b * g.scb;
+    constexpr int PAD = 8;
+    __shared__ __attribute__((aligned(16))) __bf16 As[NSPLIT][BM][BK + PAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NSPLIT][BN][BK + PAD];
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    constexpr int WM = BM / 2, WN = BN / 2, TI = WM / 16, TJ = WN / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    const int m0 = tm * BM, n0 = tn * BN;
+    f4 acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
+    double ra[NA], rb[NB];
+    const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
+    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, g.K, ra);
+    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, g.K, rb);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NA; ++s) {
+            int r, k;
+            panel_coord<BM, BK>(s, akc, r, k);
+            double x = ra[s];
+#pragma unroll
+            for (int t = 0; t < NSPLIT; ++t) {
+                __bf16 h = (__bf16)(float)x;
+                As[t][r][k] = h;
+                x -= (double)(float)h;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NB; ++s) {
+            int r, k;
+            panel_coord<BN, BK>(s, bkc, r, k);
+            double x = rb[s];
+#pragma unroll
+            for (int t = 0; t < NSPLIT; ++t) {
+                __bf16 h = (__bf16)(float)x;
+                Bs[t][r][k] = h;
+                x -= (double)(float)h;
+            }
+        }
+        __syncthreads();
+        if (k0 + BK < g.K) {
+            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, g.K, ra);
+            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, g.K, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 32) {
+            bf8 af[NSPLIT][TI], bfr[NSPLIT][TJ];
+#pragma unroll
+            for (int t = 0; t < NSPLIT; ++t) {
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+                    af[t][i] = *(const bf8 *)&As[t][wm + 16 * i + lane % 16][kk + 8 * (lane / 16)];
+#pragma unroll
+                for (int j = 0; j < TJ; ++j)
+                    bfr[t][j] = *(const bf8 *)&Bs[t][wn + 16 * j + lane % 16][kk + 8 * (lane / 16)];
+            }
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    // small cross terms first, the hi*hi product last
+                    if constexpr (NSPLIT == 3) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][i], bfr[0][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfr[1][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[2][j], acc[i][j], 0, 0, 0);
+                    }
+                    if constexpr (NSPLIT >= 2) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfr[0][j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[1][j], acc[i][j], 0, 0, 0);
+                    }
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[0][j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+    const double alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = m0 + wm + 16 * i + 4 * (lane / 16) + v;
+                int c = n0 + wn + 16 * j + lane % 16;
+                if (r < g.M && c < g.N) {
+                    int64_t o = r * g.scm + c * g.scn;
+                    double val = alpha * (double)acc[i][j][v];
+                    if (beta != 0.0) val += beta * (double)C[o];
+                    C[o] = (TIN)val;
+                }
+            }
+}
+
+template <typename TIN>
+static int launch_gemm(GemmArgs &a, int precision, hipStream_t s) {
+    int total = a.tile_start[a.n_groups];
+    if (total == 0) return GTN_OK;
+    int grid = (total + 7) / 8 * 8;
+    switch (precision) {
+        case GTN_PREC_F64:
+            hipLaunchKernelGGL((k_gemm_fp<TIN, double, 64, 64, 16>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case GTN_PREC_F32:
+            hipLaunchKernelGGL((k_gemm_fp<TIN, float, 64, 64, 16>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case GTN_PREC_BF16X3:
+            hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case GTN_PREC_BF16X6:
+            hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        default: set_error("gtn_gemm_grouped: unknown precision %d", precision); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+}  // namespace gtn
+
+using namespace gtn;
+
+extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream) {
+    GTN_REQUIRE(n_groups >= 0 && (n_groups == 0 || groups), "gtn_gemm_grouped: bad groups");
+    GTN_REQUIRE(dtype == GTN_F64 || dtype == GTN_F32, "gtn_gemm_grouped: dtype must be F64 or F32");
+    const int BM = 64, BN = 64;
+    hipStream_t s = as_stream(stream);
+    for (int base = 0; base < n_groups; base += kMaxGroups) {
+        GemmArgs a;
+        memset(&a, 0, sizeof(a));
+        int n = n_groups - base < kMaxGroups ? n_groups - base : kMaxGroups;
+        int tiles = 0;
+        int used = 0;
+        for (int i = 0; i < n; ++i) {
+            const gtn_gemm_desc &d = groups[base + i];
+            GTN_REQUIRE(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 0, "gtn_gemm_grouped: negative size");
+            GTN_REQUIRE(d.M < (1 << 30) && d.N < (1 << 30) && d.K < (1 << 30), "gtn_gemm_grouped: size too large");
+            GTN_REQUIRE(d.sak == 1 || d.sam == 1 || d.M <= 1 || d.K <= 1, "gtn_gemm_grouped: A needs a unit stride");
+            GTN_REQUIRE(d.sbk == 1 || d.sbn == 1 || d.N <= 1 || d.K <= 1, "gtn_gemm_grouped: B needs a unit stride");
+            if (d.M == 0 || d.N == 0 || d.batch == 0) continue;
+            GTN_REQUIRE(d.A && d.B && d.C, "gtn_gemm_grouped: null operand");
+            GemmGroup &g = a.g[used];
+            g.A = d.A;
+            g.B = d.B;
+            g.C = d.C;
+            g.M = (int32_t)d.M;
+            g.N = (int32_t)d.N;
+            g.K = (int32_t)d.K;
+            g.batch = (int32_t)d.batch;
+            g.tiles_m = (int32_t)((d.M + BM - 1) / BM);
+            g.tiles_n = (int32_t)((d.N + BN - 1) / BN);
+            // a unit stride on a length-1 dimension is ambiguous; normalise so the loader picks
+            // the contiguous mapping
+            g.sam = d.sam;
+            g.sak = d.sak;
+            if (d.K <= 1) g.sak = 1;
+            g.sbk = d.sbk;
+            g.sbn = d.sbn;
+            if (d.K <= 1) g.sbk = 1;
+            g.scm = d.scm;
+            g.scn = d.scn;
+            g.sab = d.sab;
+            g.sbb = d.sbb;
+            g.scb = d.scb;
+            g.alpha = d.alpha;
+            g.beta = d.beta;
+            a.tile_start[used] = tiles;
+            int64_t t = (int64_t)g.tiles_m * g.tiles_n * g.batch;
+            GTN_REQUIRE(tiles + t < (1ll << 31), "gtn_gemm_grouped: too many tiles");
+            tiles += (int)t;
+            ++used;
+        }
+        a.n_groups = used;
+        a.tile_start[used] = tiles;
+        if (used == 0) continue;
+        if (dry_run()) {
+            const int es = dtype == GTN_F64 ? 8 : 4;
+            for (int i = 0; i < used; ++i) {
+                const GemmGroup &q = a.g[i];
+                int64_t lastb = q.batch - 1;
+                int64_t ea = q.K > 0 ? lastb * q.sab + (int64_t)(q.M - 1) * q.sam + (int64_t)(q.K - 1) * q.sak + 1 : 0;
+                int64_t eb = q.K > 0 ? lastb * q.sbb + (int64_t)(q.K - 1) * q.sbk + (int64_t)(q.N - 1) * q.sbn + 1 : 0;
+                int64_t ec = lastb * q.scb + (int64_t)(q.M - 1) * q.scm + (int64_t)(q.N - 1) * q.scn + 1;
+                GTN_RANGE(q.A, ea * es, "gemm A");
+                GTN_RANGE(q.B, eb * es, "gemm B");
+                GTN_RANGE(q.C, ec * es, "gemm C");
+            }
+            continue;
+        }
+        int rc = dtype == GTN_F64 ? launch_gemm<double>(a, precision, s) : launch_gemm<float>(a, precision, s);
+        if (rc) return rc;
+    }
+    return GTN_OK;
+}

@@ -1,0 +1,613 @@
+// Integer index maps and sign factors on the device (param.py:61-83; __init__.py:1011-1154,
+// 1944-2126, 2947-3170), and the FP64 reductions / elementwise helpers of the TRG step.
+#include "gtn_common.h"
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <utility>
+
+namespace gtn {
+
+static thread_local char g_err[1024] = "";
+static std::vector<std::pair<uint64_t, uint64_t>> g_regions;
+static bool g_dry = false;
+bool dry_run() { return g_dry; }
+int check_range(const void *p, int64_t bytes, const char *what) {
+    if (!g_dry || bytes <= 0) return GTN_OK;
+    uint64_t lo = (uint64_t)p, hi = lo + (uint64_t)bytes;
+    for (auto &r : g_regions)
+        if (r.first <= lo && hi <= r.first + r.second) return GTN_OK;
+    set_error("dry run: %s touches [%#llx, %#llx) outside every registered region", what, (unsigned long long)lo,
+              (unsigned long long)hi);
+    return GTN_EINVAL;
+}
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+// ---------------------------------------------------------------------------------------------
+// index tables
+// ---------------------------------------------------------------------------------------------
+__global__ void k_index_tables(int64_t n, int32_t *popc, int8_t *sigma, int32_t *enc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t u = (uint64_t)i;
+        if (popc) popc[i] = (int32_t)popc64(u);
+        if (sigma) sigma[i] = sigma_bit(u) ? (int8_t)-1 : (int8_t)1;
+        if (enc) enc[i] = (int32_t)enc_switch(u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// signed gather.  The descriptor lives in kernel-argument memory, so every lane reads the same
+// words.  Division by an axis dimension is a shift when the dimension is a power of two (all
+// fermionic legs are) and a true division otherwise.
+// ---------------------------------------------------------------------------------------------
+constexpr int AX = GTN_PERM_MAX_AX, NC = GTN_PERM_MAX_COMP;
+struct PermArgs {
+    int32_t n_out, n_virt, n_comp, n_in;
+    int32_t virt_enc_mask, virt_sigma_mask, comp_sigma_mask, in_enc_mask, in_sigma_mask, in_split_mask;
+    uint32_t out_dim[AX];
+    int8_t out_shift[AX];
+    int8_t out_virt[AX][2];
+    int64_t out_vmul[AX][2];
+    int8_t virt_ncomp[AX];
+    uint32_t comp_dim[NC];
+    int8_t comp_shift[NC];
+    int8_t comp_bit[NC];
+    int8_t comp_in[NC][2];
+    int64_t comp_mult[NC][2];
+    int64_t in_stride[AX];
+    int64_t in_stride_lo[AX];
+    uint32_t table[2 * (1 << GTN_PERM_MAX_BITS) / 32];
+    double alpha_re, alpha_im;
+    int64_t total;
+};
+
+// Returns the source offset of output element `lin` and its table entry (0: +1, 1: -1, 2: zero).
+__device__ __forceinline__ int64_t perm_map(const PermArgs &a, uint64_t lin, uint32_t &code) {
+    uint64_t V[AX];
+    uint64_t F[AX];
+#pragma unroll
+    for (int b = 0; b < AX; ++b) {
+        V[b] = 0;
+        F[b] = 0;
+    }
+    uint64_t rest = lin;
+    for (int ax = a.n_out - 1; ax >= 0; --ax) {
+        uint64_t d = a.out_dim[ax], o;
+        if (a.out_shift[ax] >= 0) {
+            o = rest & (d - 1);
+            rest >>= a.out_shift[ax];
+        } else {
+            o = rest % d;
+            rest /= d;
+        }
+        if (a.out_virt[ax][0] >= 0) V[a.out_virt[ax][0]] += o * (uint64_t)a.out_vmul[ax][0];
+        if (a.out_virt[ax][1] >= 0) V[a.out_virt[ax][1]] += o * (uint64_t)a.out_vmul[ax][1];
+    }
+    uint32_t bits = 0, neg = 0;
+    int j = a.n_comp;
+    for (int v = a.n_virt - 1; v >= 0; --v) {
+        uint64_t c = ((a.virt_enc_mask >> v) & 1) ? enc_switch(V[v]) : V[v];
+        if ((a.virt_sigma_mask >> v) & 1) neg ^= sigma_bit(c);
+        for (int t = 0; t < a.virt_ncomp[v]; ++t) {
+            --j;
+            uint64_t d = a.comp_dim[j], k;
+            if (a.comp_shift[j] >= 0) {
+                k = c & (d - 1);
+                c >>= a.comp_shift[j];
+            } else {
+                k = c % d;
+                c /= d;
+            }
+            uint32_t pc = popc64(k);
+            if (a.comp_bit[j] >= 0) bits ^= (pc & 1u) << a.comp_bit[j];
+            if ((a.comp_sigma_mask >> j) & 1) neg ^= (pc >> 1) & 1u;
+            F[a.comp_in[j][0]] += k * (uint64_t)a.comp_mult[j][0];
+            if (a.comp_in[j][1] >= 0) F[a.comp_in[j][1]] += k * (uint64_t)a.comp_mult[j][1];
+        }
+    }
+    int64_t off = 0;
+    for (int b = 0; b < a.n_in; ++b) {
+        uint64_t f = F[b];
+        if ((a.in_sigma_mask >> b) & 1) neg ^= sigma_bit(f);
+        uint64_t e = ((a.in_enc_mask >> b) & 1) ? enc_switch(f) : f;
+        if ((a.in_split_mask >> b) & 1)
+            off += (int64_t)(e & 1) * a.in_stride_lo[b] + (int64_t)(e >> 1) * a.in_stride[b];
+        else
+            off += (int64_t)e * a.in_stride[b];
+    }
+    uint32_t t = (a.table[bits >> 4] >> ((bits & 15u) * 2)) & 3u;
+    code = (t == 2u) ? 2u : (t ^ neg);
+    return off;
+}
+
+template <typename T>
+__global__ void k_permute(PermArgs a, const T *__restrict__ src, T *__restrict__ dst) {
+    const double al = a.alpha_re;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.total; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t code;
+        int64_t off = perm_map(a, (uint64_t)i, code);
+        double v = (code == 2u) ? 0.0 : (double)src[off] * (code ? -al : al);
+        dst[i] = (T)v;
+    }
+}
+
+__global__ void k_permute_c128(PermArgs a, const double2 *__restrict__ src, double2 *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.total; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t code;
+        int64_t off = perm_map(a, (uint64_t)i, code);
+        double2 r = {0.0, 0.0};
+        if (code != 2u) {
+            double2 v = src[off];
+            double s = code ? -1.0 : 1.0;
+            r.x = s * (a.alpha_re * v.x - a.alpha_im * v.y);
+            r.y = s * (a.alpha_re * v.y + a.alpha_im * v.x);
+        }
+        dst[i] = r;
+    }
+}
+
+static int ilog2_exact(uint64_t d) {
+    if (d == 0 || (d & (d - 1))) return -1;
+    int s = 0;
+    while ((1ull << s) < d) ++s;
+    return s;
+}
+
+static int build_perm_args(const gtn_perm_desc *d, PermArgs &a) {
+    GTN_REQUIRE(d->n_out >= 0 && d->n_out <= AX, "gtn_permute: n_out out of range");
+    GTN_REQUIRE(d->n_virt >= 0 && d->n_virt <= AX, "gtn_permute: n_virt out of range");
+    GTN_REQUIRE(d->n_comp >= 0 && d->n_comp <= NC, "gtn_permute: n_comp out of range");
+    GTN_REQUIRE(d->n_in >= 0 && d->n_in <= AX, "gtn_permute: n_in out of range");
+    GTN_REQUIRE(d->n_bits >= 0 && d->n_bits <= GTN_PERM_MAX_BITS, "gtn_permute: n_bits out of range");
+    memset(&a, 0, sizeof(a));
+    a.n_out = d->n_out;
+    a.n_virt = d->n_virt;
+    a.n_comp = d->n_comp;
+    a.n_in = d->n_in;
+    int64_t total = 1;
+    int64_t vmax[AX] = {0};
+    for (int ax = 0; ax < d->n_out; ++ax) {
+        GTN_REQUIRE(d->out_dim[ax] >= 0 && d->out_dim[ax] < (1ll << 31), "gtn_permute: bad out_dim");
+        total *= d->out_dim[ax];
+        a.out_dim[ax] = (uint32_t)d->out_dim[ax];
+        a.out_shift[ax] = (int8_t)ilog2_exact((uint64_t)d->out_dim[ax]);
+        for (int t = 0; t < 2; ++t) {
+            int v = d->out_virt[ax][t];
+            GTN_REQUIRE(v >= -1 && v < d->n_virt, "gtn_permute: out_virt out of range");
+            GTN_REQUIRE(d->out_vmul[ax][t] >= 0, "gtn_permute: negative out_vmul");
+            a.out_virt[ax][t] = (int8_t)v;
+            a.out_vmul[ax][t] = d->out_vmul[ax][t];
+            if (v >= 0 && d->out_dim[ax] > 0) vmax[v] += (d->out_dim[ax] - 1) * d->out_vmul[ax][t];
+        }
+    }
+    int jc = 0;
+    int64_t fmax[AX] = {0};
+    for (int v = 0; v < d->n_virt; ++v) {
+        GTN_REQUIRE(d->virt_dim[v] >= 1, "gtn_permute: bad virt_dim");
+        GTN_REQUIRE(vmax[v] < d->virt_dim[v], "gtn_permute: virtual axis %d overflows", v);
+        if (d->virt_enc[v]) {
+            GTN_REQUIRE(ilog2_exact((uint64_t)d->virt_dim[v]) >= 0, "gtn_permute: encoder on a non power-of-two axis");
+            a.virt_enc_mask |= 1 << v;
+        }
+        if (d->virt_sigma[v]) a.virt_sigma_mask |= 1 << v;
+        a.virt_ncomp[v] = (int8_t)d->virt_ncomp[v];
+        int64_t prod = 1;
+        for (int t = 0; t < d->virt_ncomp[v]; ++t, ++jc) {
+            GTN_REQUIRE(jc < d->n_comp, "gtn_permute: components do not add up");
+            prod *= d->comp_dim[jc];
+        }
+        GTN_REQUIRE(prod == d->virt_dim[v], "gtn_permute: components of virtual axis %d do not multiply to its dim", v);
+    }
+    GTN_REQUIRE(jc == d->n_comp, "gtn_permute: components do not add up");
+    for (int j = 0; j < d->n_comp; ++j) {
+        GTN_REQUIRE(d->comp_dim[j] >= 1 && d->comp_dim[j] < (1ll << 31), "gtn_permute: bad comp_dim");
+        a.comp_dim[j] = (uint32_t)d->comp_dim[j];
+        a.comp_shift[j] = (int8_t)ilog2_exact((uint64_t)d->comp_dim[j]);
+        GTN_REQUIRE(d->comp_bit[j] >= -1 && d->comp_bit[j] < d->n_bits, "gtn_permute: comp_bit beyond n_bits");
+        a.comp_bit[j] = (int8_t)d->comp_bit[j];
+        if (d->comp_sigma[j]) a.comp_sigma_mask |= 1 << j;
+        GTN_REQUIRE(d->comp_in[j][0] >= 0 && d->comp_in[j][0] < d->n_in, "gtn_permute: comp_in out of range");
+        GTN_REQUIRE(d->comp_in[j][1] >= -1 && d->comp_in[j][1] < d->n_in, "gtn_permute: comp_in out of range");
+        for (int t = 0; t < 2; ++t) {
+            a.comp_in[j][t] = (int8_t)d->comp_in[j][t];
+            a.comp_mult[j][t] = d->comp_mult[j][t];
+            GTN_REQUIRE(d->comp_mult[j][t] >= 0, "gtn_permute: negative comp_mult");
+            if (d->comp_in[j][t] >= 0) fmax[d->comp_in[j][t]] += (d->comp_dim[j] - 1) * d->comp_mult[j][t];
+        }
+    }
+    // bounds: every fused input index stays inside its axis, the furthest element inside src
+    int64_t far = 0;
+    for (int b = 0; b < d->n_in; ++b) {
+        GTN_REQUIRE(fmax[b] < d->in_dim[b], "gtn_permute: input axis %d overflows (%lld >= %lld)", b, (long long)fmax[b],
+                    (long long)d->in_dim[b]);
+        if (d->in_enc[b]) {
+            GTN_REQUIRE(ilog2_exact((uint64_t)d->in_dim[b]) >= 0, "gtn_permute: encoder on a non power-of-two input axis");
+            a.in_enc_mask |= 1 << b;
+        }
+        if (d->in_sigma[b]) a.in_sigma_mask |= 1 << b;
+        GTN_REQUIRE(d->in_stride[b] >= 0 && d->in_stride_lo[b] >= 0, "gtn_permute: negative stride");
+        a.in_stride[b] = d->in_stride[b];
+        a.in_stride_lo[b] = d->in_stride_lo[b];
+        // the largest encoded index is in_dim - 1 (eps maps [0, 2^n) onto itself)
+        int64_t emax = d->in_dim[b] - 1;
+        if (d->in_split[b]) {
+            a.in_split_mask |= 1 << b;
+            GTN_REQUIRE(d->in_dim[b] % 2 == 0 || d->in_dim[b] == 1, "gtn_permute: split axis needs an even dim");
+            far += (emax >= 1 ? d->in_stride_lo[b] : 0) + (emax >> 1) * d->in_stride[b];
+        } else {
+            far += emax * d->in_stride[b];
+        }
+    }
+    GTN_REQUIRE(total == 0 || far < d->src_elems, "gtn_permute: source too small (%lld needed, %lld given)",
+                (long long)far + 1, (long long)d->src_elems);
+    memcpy(a.table, d->table, sizeof(a.table));
+    a.alpha_re = d->alpha_re;
+    a.alpha_im = d->alpha_im;
+    a.total = total;
+    return GTN_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// sign tensor (diagnostic output of the reference's S1 / S3)
+// ---------------------------------------------------------------------------------------------
+struct SignArgs {
+    int32_t ndim;
+    uint32_t dims[GTN_PERM_MAX_BITS];
+    uint32_t table[(1 << GTN_PERM_MAX_BITS) / 32];
+    int64_t total;
+};
+
+__global__ void k_sign_tensor(SignArgs a, int8_t *out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.total; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t r = (uint64_t)i;
+        uint32_t bits = 0;
+        for (int ax = a.ndim - 1; ax >= 0; --ax) {
+            uint64_t k = r % a.dims[ax];
+            r /= a.dims[ax];
+            bits |= (popc64(k) & 1u) << ax;
+        }
+        out[i] = ((a.table[bits >> 5] >> (bits & 31u)) & 1u) ? (int8_t)-1 : (int8_t)1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// reductions (FP64 accumulation, deterministic two-pass: per-block partials then one block)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double sq(T v) { return (double)v * (double)v; }
+template <>
+__device__ __forceinline__ double sq<double2>(double2 v) { return v.x * v.x + v.y * v.y; }
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    }
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+__global__ void k_sumsq_partial(const T *__restrict__ x, int64_t n, double *part) {
+    __shared__ double sh[16];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += sq(x[i]);
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_sum_final(const double *part, int np, double *out) {
+    __shared__ double sh[16];
+    double acc = 0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+template <typename T>
+__global__ void k_scale_by_norm(T *x, int64_t n, const double *nrm2) {
+    double inv = 1.0 / sqrt(nrm2[0]);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (T)((double)x[i] * inv);
+}
+__global__ void k_scale_by_norm_c(double2 *x, int64_t n, const double *nrm2) {
+    double inv = 1.0 / sqrt(nrm2[0]);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = x[i];
+        v.x *= inv;
+        v.y *= inv;
+        x[i] = v;
+    }
+}
+
+template <typename T>
+__global__ void k_reduce_last(const T *__restrict__ src, T *__restrict__ dst, int64_t rows, int64_t inner) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0;
+        for (int64_t t = 0; t < inner; ++t) acc += (double)src[r * inner + t];
+        dst[r] = (T)acc;
+    }
+}
+__global__ void k_reduce_last_c(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t rows, int64_t inner) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = {0, 0};
+        for (int64_t t = 0; t < inner; ++t) {
+            acc.x += src[r * inner + t].x;
+            acc.y += src[r * inner + t].y;
+        }
+        dst[r] = acc;
+    }
+}
+
+__global__ void k_lognorm(double *ln, const double *nrm2) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ln[0] = 2.0 * ln[0] + 0.5 * log(nrm2[0]);
+}
+
+template <typename T>
+__global__ void k_power_diag(T *x, int64_t n, int64_t stride, double p, double rcond) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = (double)x[i * stride];
+        x[i * stride] = (T)(fabs(v) > rcond ? pow(v, p) : 0.0);
+    }
+}
+
+struct OddArgs {
+    int32_t ndim;
+    uint32_t dims[GTN_PERM_MAX_AX];
+    int32_t ferm_mask, pp_mask;
+};
+template <typename T>
+__global__ void k_odd_partial(OddArgs a, const T *__restrict__ x, int64_t n, double *part) {
+    __shared__ double sh[16];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t r = (uint64_t)i;
+        uint32_t par = 0;
+        for (int ax = a.ndim - 1; ax >= 0; --ax) {
+            uint64_t k = r % a.dims[ax];
+            r /= a.dims[ax];
+            if ((a.ferm_mask >> ax) & 1) par ^= ((a.pp_mask >> ax) & 1) ? (uint32_t)(k & 1) : (popc64(k) & 1u);
+        }
+        if (par) acc += sq(x[i]);
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename T>
+__global__ void k_axpby(const T *x, const T *y, T *out, int64_t n, double a, double b) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (T)(a * (double)x[i] + (y ? b * (double)y[i] : 0.0));
+}
+__global__ void k_axpby_c(const double2 *x, const double2 *y, double2 *out, int64_t n, double ar, double ai, double br,
+                          double bi) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 u = x[i], r;
+        r.x = ar * u.x - ai * u.y;
+        r.y = ar * u.y + ai * u.x;
+        if (y) {
+            double2 v = y[i];
+            r.x += br * v.x - bi * v.y;
+            r.y += br * v.y + bi * v.x;
+        }
+        out[i] = r;
+    }
+}
+
+static double *g_partials = nullptr;  // 1024 doubles per device, grown on demand
+static int g_partials_dev = -1;
+static int partial_buffer(double **p) {
+    int dev;
+    GTN_CHECK_HIP(hipGetDevice(&dev));
+    if (!g_partials || g_partials_dev != dev) {
+        GTN_CHECK_HIP(hipMalloc(&g_partials, 1024 * sizeof(double)));
+        g_partials_dev = dev;
+    }
+    *p = g_partials;
+    return GTN_OK;
+}
+
+}  // namespace gtn
+
+using namespace gtn;
+
+extern "C" {
+
+int gtn_version(void) { return 100; }
+
+int gtn_set_dry_run(int n, const uint64_t *bases, const int64_t *bytes) {
+    g_regions.clear();
+    for (int i = 0; i < n; ++i) g_regions.push_back({bases[i], (uint64_t)bytes[i]});
+    g_dry = n > 0;
+    return GTN_OK;
+}
+const char *gtn_last_error(void) { return gtn::g_err; }
+
+int gtn_device_info(char *name, int name_len, int *n_cu) {
+    int dev;
+    GTN_CHECK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    GTN_CHECK_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) snprintf(name, name_len, "%s", p.gcnArchName);
+    if (n_cu) *n_cu = p.multiProcessorCount;
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+        set_error("gtn_device_info: device is %s, this library is built for gfx950", p.gcnArchName);
+        return GTN_ENOTSUP;
+    }
+    return GTN_OK;
+}
+
+int gtn_index_tables(int64_t n, int32_t *popc, int8_t *sigma, int32_t *enc, void *stream) {
+    GTN_REQUIRE(n >= 0, "gtn_index_tables: n < 0");
+    if (n == 0) return GTN_OK;
+    GTN_RANGE(popc, popc ? n * 4 : 0, "index_tables popc");
+    GTN_RANGE(sigma, sigma ? n : 0, "index_tables sigma");
+    GTN_RANGE(enc, enc ? n * 4 : 0, "index_tables enc");
+    if (dry_run()) return GTN_OK;
+    hipLaunchKernelGGL(k_index_tables, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), n, popc, sigma, enc);
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream) {
+    GTN_REQUIRE(d && src && dst, "gtn_permute: null argument");
+    PermArgs a;
+    int rc = build_perm_args(d, a);
+    if (rc) return rc;
+    if (a.total == 0) return GTN_OK;
+    if (dry_run()) {
+        int es = d->dtype == GTN_C128 ? 16 : (d->dtype == GTN_F32 ? 4 : 8);
+        GTN_RANGE(src, d->src_elems * es, "permute src");
+        GTN_RANGE(dst, a.total * es, "permute dst");
+        return GTN_OK;
+    }
+    dim3 g(grid_for(a.total, 256, 1 << 20)), b(256);
+    hipStream_t s = as_stream(stream);
+    switch (d->dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_permute<double>, g, b, 0, s, a, (const double *)src, (double *)dst); break;
+        case GTN_F32: hipLaunchKernelGGL(k_permute<float>, g, b, 0, s, a, (const float *)src, (float *)dst); break;
+        case GTN_C128: hipLaunchKernelGGL(k_permute_c128, g, b, 0, s, a, (const double2 *)src, (double2 *)dst); break;
+        default: set_error("gtn_permute: unsupported dtype %d", d->dtype); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_sign_tensor(int ndim, const int64_t *dims, const uint32_t *table, int8_t *out, void *stream) {
+    GTN_REQUIRE(ndim >= 0 && ndim <= GTN_PERM_MAX_BITS, "gtn_sign_tensor: ndim out of range");
+    SignArgs a;
+    memset(&a, 0, sizeof(a));
+    a.ndim = ndim;
+    int64_t total = 1;
+    for (int i = 0; i < ndim; ++i) {
+        GTN_REQUIRE(dims[i] >= 1 && dims[i] < (1ll << 31), "gtn_sign_tensor: bad dim");
+        a.dims[i] = (uint32_t)dims[i];
+        total *= dims[i];
+    }
+    a.total = total;
+    memcpy(a.table, table, ((1u << ndim) + 31) / 32 * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_sign_tensor, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0, as_stream(stream), a, out);
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_sumsq(const void *x, int64_t n, int dtype, double *out, void *stream) {
+    double *part;
+    int rc = partial_buffer(&part);
+    if (rc) return rc;
+    int g = grid_for(n, 256, 1024);
+    hipStream_t s = as_stream(stream);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_sumsq_partial<double>, dim3(g), dim3(256), 0, s, (const double *)x, n, part); break;
+        case GTN_F32: hipLaunchKernelGGL(k_sumsq_partial<float>, dim3(g), dim3(256), 0, s, (const float *)x, n, part); break;
+        case GTN_C128: hipLaunchKernelGGL(k_sumsq_partial<double2>, dim3(g), dim3(256), 0, s, (const double2 *)x, n, part); break;
+        default: set_error("gtn_sumsq: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(1024), 0, s, (const double *)part, g, out);
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_scale_by_norm(void *x, int64_t n, int dtype, const double *nrm2, void *stream) {
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(n, 256, 1 << 16)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_scale_by_norm<double>, g, b, 0, s, (double *)x, n, nrm2); break;
+        case GTN_F32: hipLaunchKernelGGL(k_scale_by_norm<float>, g, b, 0, s, (float *)x, n, nrm2); break;
+        case GTN_C128: hipLaunchKernelGGL(k_scale_by_norm_c, g, b, 0, s, (double2 *)x, n, nrm2); break;
+        default: set_error("gtn_scale_by_norm: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_reduce_last(const void *src, void *dst, int64_t rows, int64_t inner, int dtype, void *stream) {
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(rows, 256, 1 << 16)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_reduce_last<double>, g, b, 0, s, (const double *)src, (double *)dst, rows, inner); break;
+        case GTN_F32: hipLaunchKernelGGL(k_reduce_last<float>, g, b, 0, s, (const float *)src, (float *)dst, rows, inner); break;
+        case GTN_C128: hipLaunchKernelGGL(k_reduce_last_c, g, b, 0, s, (const double2 *)src, (double2 *)dst, rows, inner); break;
+        default: set_error("gtn_reduce_last: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_lognorm_update(double *ln, const double *nrm2, void *stream) {
+    hipLaunchKernelGGL(k_lognorm, dim3(1), dim3(64), 0, as_stream(stream), ln, nrm2);
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_power_diag(void *x, int64_t n, int64_t stride, int dtype, double p, double rcond, void *stream) {
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(n, 256)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_power_diag<double>, g, b, 0, s, (double *)x, n, stride, p, rcond); break;
+        case GTN_F32: hipLaunchKernelGGL(k_power_diag<float>, g, b, 0, s, (float *)x, n, stride, p, rcond); break;
+        default: set_error("gtn_power_diag: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_axpby(const void *x, const void *y, void *out, int64_t n, int dtype, double ar, double ai, double br, double bi,
+              void *stream) {
+    if (n == 0) return GTN_OK;
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(n, 256, 1 << 16)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_axpby<double>, g, b, 0, s, (const double *)x, (const double *)y, (double *)out, n, ar, br); break;
+        case GTN_F32: hipLaunchKernelGGL(k_axpby<float>, g, b, 0, s, (const float *)x, (const float *)y, (float *)out, n, ar, br); break;
+        case GTN_C128:
+            hipLaunchKernelGGL(k_axpby_c, g, b, 0, s, (const double2 *)x, (const double2 *)y, (double2 *)out, n, ar, ai, br, bi);
+            break;
+        default: set_error("gtn_axpby: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_odd_norm(const void *x, int ndim, const int64_t *dims, const int32_t *ferm, const int32_t *pp, int dtype,
+                 double *out, void *stream) {
+    GTN_REQUIRE(ndim >= 0 && ndim <= GTN_PERM_MAX_AX, "gtn_odd_norm: ndim out of range");
+    OddArgs a;
+    memset(&a, 0, sizeof(a));
+    a.ndim = ndim;
+    int64_t n = 1;
+    for (int i = 0; i < ndim; ++i) {
+        a.dims[i] = (uint32_t)dims[i];
+        n *= dims[i];
+        if (ferm[i]) a.ferm_mask |= 1 << i;
+        if (pp[i]) a.pp_mask |= 1 << i;
+    }
+    double *part;
+    int rc = partial_buffer(&part);
+    if (rc) return rc;
+    int g = grid_for(n, 256, 1024);
+    hipStream_t s = as_stream(stream);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_odd_partial<double>, dim3(g), dim3(256), 0, s, a, (const double *)x, n, part); break;
+        case GTN_F32: hipLaunchKernelGGL(k_odd_partial<float>, dim3(g), dim3(256), 0, s, a, (const float *)x, n, part); break;
+        case GTN_C128: hipLaunchKernelGGL(k_odd_partial<double2>, dim3(g), dim3(256), 0, s, a, (const double2 *)x, n, part); break;
+        default: set_error("gtn_odd_norm: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(1024), 0, s, (const double *)part, g, out);
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+}  // extern "C"

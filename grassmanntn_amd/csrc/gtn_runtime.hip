@@ -1,0 +1,179 @@
+// Device memory, copies, fills, events and synchronisation for the Python host layer.
+//
+// The library owns its HBM through a small caching allocator (exact size classes rounded to
+// 256 bytes, reused in stream order on the one stream the host layer uses), so that processes using
+// grassmanntn_amd need no other GPU framework and run on the HIP runtime this library was built
+// against.
+#include "gtn_common.h"
+#include <map>
+#include <mutex>
+#include <string.h>
+#include <vector>
+
+namespace gtn {
+
+static std::mutex g_mu;
+static std::map<int64_t, std::vector<void *>> g_free;  // size class -> cached blocks
+static std::map<void *, int64_t> g_live;               // pointer -> size class
+static int64_t g_cached_bytes = 0;
+static const int64_t kCacheCap = 64ll << 30;          // release cached blocks beyond 64 GiB
+
+template <typename T>
+__global__ void k_fill_value(T *p, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <typename T>
+__global__ void k_copy_2d(const T *__restrict__ src, int64_t lds, T *__restrict__ dst, int64_t ldd, int64_t rows,
+                          int64_t cols) {
+    int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = e / cols, c = e - r * cols;
+        dst[r * ldd + c] = src[r * lds + c];
+    }
+}
+
+}  // namespace gtn
+
+using namespace gtn;
+
+extern "C" {
+
+int gtn_malloc(int64_t bytes, void **out) {
+    GTN_REQUIRE(out && bytes >= 0, "gtn_malloc: bad argument");
+    int64_t cls = (bytes + 255) / 256 * 256;
+    if (cls == 0) cls = 256;
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        auto it = g_free.find(cls);
+        if (it != g_free.end() && !it->second.empty()) {
+            *out = it->second.back();
+            it->second.pop_back();
+            g_cached_bytes -= cls;
+            g_live[*out] = cls;
+            return GTN_OK;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)cls);
+    if (e != hipSuccess) {
+        // give the cache back to the device and retry once
+        std::lock_guard<std::mutex> l(g_mu);
+        for (auto &kv : g_free)
+            for (void *q : kv.second) (void)hipFree(q);
+        g_free.clear();
+        g_cached_bytes = 0;
+        e = hipMalloc(&p, (size_t)cls);
+    }
+    if (e != hipSuccess) {
+        set_error("gtn_malloc: hipMalloc(%lld) failed: %s", (long long)cls, hipGetErrorString(e));
+        return GTN_ENOMEM;
+    }
+    std::lock_guard<std::mutex> l(g_mu);
+    g_live[p] = cls;
+    *out = p;
+    return GTN_OK;
+}
+
+int gtn_free(void *p) {
+    if (!p) return GTN_OK;
+    std::lock_guard<std::mutex> l(g_mu);
+    auto it = g_live.find(p);
+    GTN_REQUIRE(it != g_live.end(), "gtn_free: pointer not allocated by gtn_malloc");
+    int64_t cls = it->second;
+    g_live.erase(it);
+    if (g_cached_bytes + cls > kCacheCap) {
+        GTN_CHECK_HIP(hipFree(p));
+        return GTN_OK;
+    }
+    g_free[cls].push_back(p);
+    g_cached_bytes += cls;
+    return GTN_OK;
+}
+
+// kind: 1 host->device, 2 device->host, 3 device->device.  Host<->device copies are synchronous;
+// device->device copies are ordered on `stream`.
+int gtn_memcpy(void *dst, const void *src, int64_t bytes, int kind, void *stream) {
+    if (bytes == 0) return GTN_OK;
+    GTN_REQUIRE(dst && src && bytes > 0, "gtn_memcpy: bad argument");
+    hipStream_t s = as_stream(stream);
+    switch (kind) {
+        case 1:
+            GTN_CHECK_HIP(hipStreamSynchronize(s));
+            GTN_CHECK_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+            break;
+        case 2:
+            GTN_CHECK_HIP(hipStreamSynchronize(s));
+            GTN_CHECK_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+            break;
+        case 3: GTN_CHECK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, s)); break;
+        default: set_error("gtn_memcpy: unknown kind %d", kind); return GTN_EINVAL;
+    }
+    return GTN_OK;
+}
+
+int gtn_fill(void *p, int64_t n, int dtype, double value, void *stream) {
+    if (n == 0) return GTN_OK;
+    GTN_REQUIRE(p && n > 0, "gtn_fill: bad argument");
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(n, 256, 1 << 16)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_fill_value<double>, g, b, 0, s, (double *)p, n, value); break;
+        case GTN_F32: hipLaunchKernelGGL(k_fill_value<float>, g, b, 0, s, (float *)p, n, (float)value); break;
+        case GTN_C128:
+            GTN_REQUIRE(value == 0.0, "gtn_fill: complex tensors can only be zero-filled");
+            hipLaunchKernelGGL(k_fill_value<double>, g, b, 0, s, (double *)p, 2 * n, 0.0);
+            break;
+        case GTN_I8: hipLaunchKernelGGL(k_fill_value<int8_t>, g, b, 0, s, (int8_t *)p, n, (int8_t)value); break;
+        default: set_error("gtn_fill: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_copy2d(void *dst, int64_t ldd, const void *src, int64_t lds, int64_t rows, int64_t cols, int dtype, void *stream) {
+    if (rows <= 0 || cols <= 0) return GTN_OK;
+    GTN_REQUIRE(dst && src && ldd >= cols && lds >= cols, "gtn_copy2d: bad argument");
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(rows * cols, 256, 1 << 16)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_copy_2d<double>, g, b, 0, s, (const double *)src, lds, (double *)dst, ldd, rows, cols); break;
+        case GTN_F32: hipLaunchKernelGGL(k_copy_2d<float>, g, b, 0, s, (const float *)src, lds, (float *)dst, ldd, rows, cols); break;
+        case GTN_C128:
+            hipLaunchKernelGGL(k_copy_2d<double2>, g, b, 0, s, (const double2 *)src, lds, (double2 *)dst, ldd, rows, cols);
+            break;
+        default: set_error("gtn_copy2d: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_synchronize(void) {
+    GTN_CHECK_HIP(hipDeviceSynchronize());
+    return GTN_OK;
+}
+
+int gtn_event_create(void **ev) {
+    hipEvent_t e;
+    GTN_CHECK_HIP(hipEventCreate(&e));
+    *ev = (void *)e;
+    return GTN_OK;
+}
+
+int gtn_event_record(void *ev, void *stream) {
+    GTN_CHECK_HIP(hipEventRecord((hipEvent_t)ev, as_stream(stream)));
+    return GTN_OK;
+}
+
+int gtn_event_elapsed_ms(void *start, void *stop, float *ms) {
+    GTN_CHECK_HIP(hipEventSynchronize((hipEvent_t)stop));
+    GTN_CHECK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return GTN_OK;
+}
+
+int gtn_event_destroy(void *ev) {
+    GTN_CHECK_HIP(hipEventDestroy((hipEvent_t)ev));
+    return GTN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+/*
+ * gtn_hip.h -- C ABI of the MI355X (gfx950) Grassmann-tensor kernels.
+ *
+ * The reference (grassmanntn) is pure Python/numpy: it has no FFI of its own.  Each entry point here
+ * replaces one numeric seam of that Python code; the comment on each names the reference function
+ * (file:line, paths relative to the grassmanntn tree) whose arithmetic it performs.  The Python
+ * package grassmanntn_amd binds these with ctypes (see grassmanntn_amd/_lib.py).
+ *
+ * Conventions
+ *   - every pointer argument named *_dev is device memory (HBM); descriptors are host memory;
+ *   - `stream` is a hipStream_t passed as void*; NULL means the default stream;
+ *   - every entry point returns 0 on success and a nonzero GTN_E* code on failure, after recording
+ *     a message that gtn_last_error() returns (thread-local);
+ *   - all index arithmetic is exact integer arithmetic; sign factors are +-1 computed on the device
+ *     from popcounts (parity p(i) = popcount(i) & 1, sigma(i) = (-1)^(p(p-1)/2) with p = popcount(i),
+ *     encoder eps(i) = i ^ (popcount(i >> 1) & 1)).
+ */
+#ifndef GTN_HIP_H
+#define GTN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GTN_OK 0
+#define GTN_EINVAL 1   /* bad argument or descriptor */
+#define GTN_EDEVICE 2  /* HIP runtime error */
+#define GTN_ENOMEM 3   /* workspace too small */
+#define GTN_ENOTSUP 4  /* unsupported combination */
+
+/* element types */
+#define GTN_F64 0
+#define GTN_F32 1
+#define GTN_C128 2
+#define GTN_I8 3
+
+/* arithmetic precisions of the contraction GEMMs */
+#define GTN_PREC_F64 0    /* FP64 MFMA, FP64 accumulation */
+#define GTN_PREC_F32 1    /* operands rounded to FP32, FP32 MFMA, FP32 accumulation */
+#define GTN_PREC_BF16X3 2 /* operands split hi+lo in BF16, 3 BF16 MFMA products, FP32 accumulation */
+#define GTN_PREC_BF16X6 3 /* operands split in 3 BF16 parts, 6 products, FP32 accumulation */
+
+int gtn_version(void);
+
+/* Host-only validation ("dry run"): while enabled, no kernel is launched and no device memory is
+ * touched; every GEMM operand range and every buffer range a kernel would access is checked
+ * against the registered regions [base, base + bytes), and the first violation makes the entry
+ * point fail with GTN_EINVAL.  Used by CPU tests to check launch geometry with placeholder
+ * addresses.  n_regions = 0 disables the mode. */
+int gtn_set_dry_run(int n_regions, const uint64_t *bases, const int64_t *bytes);
+const char *gtn_last_error(void);
+/* Checks that the current device is a gfx950; fills its architecture name and compute-unit count. */
+int gtn_device_info(char *name, int name_len, int *n_cu);
+
+/* ------------------------------------------------------------------------------------------------
+ * Device memory and ordering.  The host layer keeps every tensor in HBM allocated here (a caching
+ * allocator: blocks freed with gtn_free are reused by later gtn_malloc calls of the same size class,
+ * in stream order on the one stream the host layer uses).
+ * gtn_memcpy kind: 1 host->device, 2 device->host (both synchronous), 3 device->device (ordered on
+ * `stream`).  gtn_fill writes `value` to n elements (complex: only 0 is accepted); gtn_copy2d copies
+ * a rows x cols block between leading dimensions lds and ldd.
+ * ---------------------------------------------------------------------------------------------- */
+int gtn_malloc(int64_t bytes, void **out_dev);
+int gtn_free(void *dev);
+int gtn_memcpy(void *dst, const void *src, int64_t bytes, int kind, void *stream);
+int gtn_fill(void *dev, int64_t n, int dtype, double value, void *stream);
+int gtn_copy2d(void *dst_dev, int64_t ldd, const void *src_dev, int64_t lds, int64_t rows, int64_t cols, int dtype,
+               void *stream);
+int gtn_synchronize(void);
+int gtn_event_create(void **ev);
+int gtn_event_record(void *ev, void *stream);
+int gtn_event_elapsed_ms(void *start, void *stop, float *ms);
+int gtn_event_destroy(void *ev);
+
+/* ------------------------------------------------------------------------------------------------
+ * (A1) index tables.  Replaces param.gparity / param.sgn / param.encoder (param.py:61-83; the
+ * literal tables of param.py:4-6 are their values on [0, 65536)).
+ * popc[i] = popcount(i), sigma[i] = +-1, enc[i] = eps(i), for 0 <= i < n.  Any pointer may be NULL.
+ * ---------------------------------------------------------------------------------------------- */
+int gtn_index_tables(int64_t n, int32_t *popc_dev, int8_t *sigma_dev, int32_t *enc_dev, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Signed gather ("permute").  One descriptor covers the reference's dense.switch_encoder
+ * (__init__.py:1070-1102), switch_format (:1011-1068), switch_parity (:1104-1154), the
+ * single-operand einsum with its S3 sign tensor (:2045-2126, :2295), the grouping signs and
+ * reshapes of join_legs / split_legs (:2947-3170), the parity-block extraction of BlockSVD
+ * (:3990-3996) and the operand preparation of every contraction (S1 / S2 folded in).
+ *
+ * Output element with raw multi-index (o_0..o_{n_out-1}) (C order, last axis fastest):
+ *   virtual axes:   V_v = sum_a o_a * out_vmul[a][t] over the (a, t) with out_virt[a][t] == v
+ *                   c_v = virt_enc[v] ? eps(V_v) : V_v                   (canonical fused index)
+ *                   sign *= virt_sigma[v] ? sigma(c_v) : 1
+ *                   c_v is split, C order, into virt_ncomp[v] consecutive components k_j
+ *   components:     bits ^= (popcount(k_j) & 1) << comp_bit[j]          (if comp_bit[j] >= 0)
+ *                   sign *= comp_sigma[j] ? sigma(k_j) : 1
+ *                   F[comp_in[j][t]] += k_j * comp_mult[j][t]           (t = 0, 1; -1 = unused)
+ *   input axes:     e_b = in_enc[b] ? eps(F_b) : F_b ;  sign *= in_sigma[b] ? sigma(F_b) : 1
+ *                   offset += in_split[b] ? (e_b & 1) * in_stride_lo[b] + (e_b >> 1) * in_stride[b]
+ *                                         : e_b * in_stride[b]
+ *   table:          2 bits per entry, entry `bits`: 0 -> +1, 1 -> -1, 2 -> the element is 0
+ *   dst[o] = alpha * sign * src[offset]   (src is not read for a zero entry)
+ * ---------------------------------------------------------------------------------------------- */
+#define GTN_PERM_MAX_AX 10
+#define GTN_PERM_MAX_COMP 16
+#define GTN_PERM_MAX_BITS 10
+typedef struct gtn_perm_desc {
+    int32_t dtype; /* GTN_F64 / GTN_F32 / GTN_C128 (src and dst share it) */
+    int32_t n_out, n_virt, n_comp, n_in, n_bits;
+    int64_t out_dim[GTN_PERM_MAX_AX];
+    int32_t out_virt[GTN_PERM_MAX_AX][2];
+    int64_t out_vmul[GTN_PERM_MAX_AX][2];
+    int64_t virt_dim[GTN_PERM_MAX_AX];
+    int32_t virt_enc[GTN_PERM_MAX_AX];
+    int32_t virt_sigma[GTN_PERM_MAX_AX];
+    int32_t virt_ncomp[GTN_PERM_MAX_AX];
+    int64_t comp_dim[GTN_PERM_MAX_COMP];
+    int32_t comp_bit[GTN_PERM_MAX_COMP];
+    int32_t comp_sigma[GTN_PERM_MAX_COMP];
+    int32_t comp_in[GTN_PERM_MAX_COMP][2];
+    int64_t comp_mult[GTN_PERM_MAX_COMP][2];
+    int64_t in_dim[GTN_PERM_MAX_AX];
+    int32_t in_enc[GTN_PERM_MAX_AX];
+    int32_t in_sigma[GTN_PERM_MAX_AX];
+    int32_t in_split[GTN_PERM_MAX_AX];
+    int64_t in_stride[GTN_PERM_MAX_AX];
+    int64_t in_stride_lo[GTN_PERM_MAX_AX];
+    uint32_t table[2 * (1 << GTN_PERM_MAX_BITS) / 32];
+    double alpha_re, alpha_im;
+    int64_t src_elems; /* number of elements addressable from src (checked on the host) */
+} gtn_perm_desc;
+int gtn_permute(const gtn_perm_desc *desc, const void *src_dev, void *dst_dev, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * (A3) sign tensor, as the reference materialises S1 / S3 (__init__.py:1944-1999, :2088-2126):
+ * out[idx] = table[bits(idx)] where bit j of bits(idx) is popcount(idx_j) & 1 (canonical indices);
+ * table: 1 bit per entry, set => -1.  Output int8 (+1 / -1), C order over dims[0..ndim).
+ * ---------------------------------------------------------------------------------------------- */
+int gtn_sign_tensor(int ndim, const int64_t *dims, const uint32_t *table, int8_t *out_dev, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * (A4/A5) grouped, batched GEMM:  C_g[b] = alpha_g * A_g[b] * B_g[b] + beta_g * C_g[b]
+ * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], C(m,n) = C[m*scm + n*scn]; batch strides
+ * sab/sbb/scb.  This is the parity-block-grouped contraction that replaces oe.contract
+ * (__init__.py:2295) and einsum_block's per-sector np.einsum (:2781): every nonzero parity sector of
+ * a contraction is one group.  Element type GTN_F64 or GTN_F32; `precision` is a GTN_PREC_* value.
+ * For each operand exactly one of its two strides must be 1.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct gtn_gemm_desc {
+    const void *A;
+    const void *B;
+    void *C;
+    int64_t M, N, K, batch;
+    int64_t sam, sak, sbk, sbn, scm, scn;
+    int64_t sab, sbb, scb;
+    double alpha, beta;
+} gtn_gemm_desc;
+int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Reductions and elementwise helpers (FP64 accumulation).
+ * gtn_sumsq: out_dev[0] = sum |x|^2 over n elements (Frobenius norm^2; dense.norm __init__.py:888-891,
+ *   gauge2d.py:1746).
+ * gtn_scale_by_norm: x /= sqrt(nrm2_dev[0]) (gauge2d.py:1747-1748).
+ * gtn_reduce_last: dst[r] = sum_{t<inner} src[r*inner + t] (traced axes of a single-operand einsum).
+ * gtn_lognorm_update: (log-norm accumulation of example.py:192) lognorm[0] = 2*lognorm[0] + 0.5*log(nrm2[0]).
+ * gtn_power_diag: matrix-format power of a diagonal with the reference's |x| > rcond rule
+ *   (power_ds, __init__.py:6059-6069): x = (|x| > rcond) ? x^p : 0, elementwise on n elements.
+ * gtn_axpby: out = alpha * x + beta * y elementwise (dense arithmetic, __init__.py:933-987); y may be
+ *   NULL (then out = alpha * x); complex tensors take complex alpha/beta.
+ * gtn_odd_norm: out_dev[0] = sum |x|^2 over entries whose fermionic parity is odd (is_grassmann_even,
+ *   __init__.py:3900-3925); parity of each axis from its index (PP: LSB, canonical: popcount).
+ * ---------------------------------------------------------------------------------------------- */
+int gtn_sumsq(const void *x_dev, int64_t n, int dtype, double *out_dev, void *stream);
+int gtn_scale_by_norm(void *x_dev, int64_t n, int dtype, const double *nrm2_dev, void *stream);
+int gtn_reduce_last(const void *src_dev, void *dst_dev, int64_t rows, int64_t inner, int dtype, void *stream);
+int gtn_lognorm_update(double *lognorm_dev, const double *nrm2_dev, void *stream);
+int gtn_power_diag(void *x_dev, int64_t n, int64_t stride, int dtype, double p, double rcond, void *stream);
+int gtn_axpby(const void *x_dev, const void *y_dev, void *out_dev, int64_t n, int dtype, double alpha_re,
+              double alpha_im, double beta_re, double beta_im, void *stream);
+int gtn_odd_norm(const void *x_dev, int ndim, const int64_t *dims, const int32_t *fermionic,
+                 const int32_t *pp, int dtype, double *out_dev, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * (A10/A11) truncated SVD / Hermitian eig of a batch of parity blocks (BlockSVD / SortedSVD,
+ * __init__.py:3931-4029; BlockEig / SortedEig :4314-4422).  For each block b (an m_b x n_b FP64
+ * matrix, row-major with leading dimension lda_b) the top k_b singular triplets are returned:
+ *   U_b (m_b x k_b, ld ldu_b), S_b (k_b), Vt_b (k_b x n_b, ld ldv_b),
+ * ordered by decreasing singular value, with singular values at or below 1e-14 relative to the
+ * largest set to zero together with their vectors (the rank rule of __init__.py:3939-3941).
+ * Algorithm: randomized subspace iteration with block width p_b = k_b + oversample, `iters`
+ * iterations, CholeskyQR2 re-orthonormalisation, then an FP64 Rayleigh-Ritz step (B = Q^T M,
+ * eigen-decomposition of B B^T by cyclic Jacobi) -- the range finder may run in a reduced
+ * `precision`; the Rayleigh-Ritz step is always FP64.
+ * Gauge: each column of U is signed so that its entry of largest magnitude is positive.
+ * eig mode (mode=1): the block must be Hermitian (checked on the host path); eigenvalues are the
+ * signed Rayleigh quotients lambda_k = u_k^T M u_k, ordered by |lambda|, and Vt = U^T.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct gtn_svd_block {
+    const double *M;
+    double *U, *S, *Vt;
+    int64_t m, n, lda, ldu, ldv;
+    int32_t k;
+} gtn_svd_block;
+typedef struct gtn_svd_opts {
+    int32_t oversample; /* extra columns in the range finder */
+    int32_t iters;      /* subspace iterations */
+    int32_t precision;  /* GTN_PREC_* for the range finder GEMMs */
+    int32_t mode;       /* 0: SVD, 1: Hermitian eig */
+    uint64_t seed;
+} gtn_svd_opts;
+int64_t gtn_svd_workspace_bytes(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts);
+int gtn_svd_blocks(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts, void *workspace_dev,
+                   int64_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

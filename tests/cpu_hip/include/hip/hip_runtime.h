@@ -1,0 +1,179 @@
+// CPU execution shim for the HIP kernels of grassmanntn_amd -- TEST INFRASTRUCTURE ONLY.
+//
+// Compiling the library's .hip sources with this directory first on the include path turns them
+// into plain C++: every GPU thread is an OS thread, a workgroup's threads run together (blocks run
+// one after another), __syncthreads is a barrier over the live threads of the block, wave-level
+// operations (__shfl_down, MFMA) exchange data through a per-wave buffer, __shared__ variables are
+// function statics (one copy, reused block after block) and dynamic LDS is a buffer whose bytes
+// beyond the requested size are poisoned for AddressSanitizer.  Built with -fsanitize=address,
+// every out-of-bounds access to global memory or LDS is reported on the CPU.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include <sanitizer/asan_interface.h>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+#define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_lds();
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct double2 {
+    double x, y;
+};
+template <typename T, unsigned N>
+struct HIP_vector_type;
+
+typedef int hipError_t;
+typedef void *hipStream_t;
+#define hipSuccess 0
+#define hipFuncAttributeMaxDynamicSharedMemorySize 8
+struct hipDeviceProp_t {
+    char gcnArchName[256];
+    int multiProcessorCount;
+};
+
+namespace emu {
+struct Idx {
+    unsigned x, y, z;
+};
+extern thread_local Idx tid, bid;
+extern Idx bdim, gdim;
+void *dyn_lds();
+void sync_block();
+void sync_wave();
+void *wave_buf(int slot);  // per-wave exchange area (64 lanes x 64 bytes per slot)
+int lane();
+}  // namespace emu
+
+#define threadIdx emu::tid
+#define blockIdx emu::bid
+#define blockDim emu::bdim
+#define gridDim emu::gdim
+
+inline void __syncthreads() { emu::sync_block(); }
+
+template <typename T>
+inline T __shfl_down(T v, unsigned delta, int width = 64) {
+    static_assert(sizeof(T) <= 8, "shuffle of a value wider than 8 bytes");
+    T *buf = (T *)emu::wave_buf(0);
+    int l = emu::lane();
+    emu::sync_wave();
+    buf[l] = v;
+    emu::sync_wave();
+    T r = (l + (int)delta < width) ? buf[l + delta] : v;
+    emu::sync_wave();
+    return r;
+}
+
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+
+// ---- MFMA emulation (wave64; operand/result layouts of the 16x16 instructions) -------------------
+typedef double emu_d4 __attribute__((ext_vector_type(4)));
+typedef float emu_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 emu_bf8 __attribute__((ext_vector_type(8)));
+
+template <typename TA, typename TC, int KPL>
+inline TC emu_mfma16(const TA *a, const TA *b, TC c) {
+    // lane l supplies A[l%16][KPL*(l/16) + t] and B[KPL*(l/16) + t][l%16], t < KPL;
+    // it receives C[4*(l/16) + v][l%16], v < 4
+    struct Slot {
+        double a[8], b[8], c[4];
+    };
+    Slot *buf = (Slot *)emu::wave_buf(1);
+    int l = emu::lane();
+    emu::sync_wave();
+    for (int t = 0; t < KPL; ++t) {
+        buf[l].a[t] = (double)a[t];
+        buf[l].b[t] = (double)b[t];
+    }
+    for (int v = 0; v < 4; ++v) buf[l].c[v] = (double)c[v];
+    emu::sync_wave();
+    TC out;
+    for (int v = 0; v < 4; ++v) {
+        int row = 4 * (l / 16) + v, col = l % 16;
+        double acc = buf[l].c[v];
+        for (int kk = 0; kk < 4 * KPL; ++kk) {
+            int qa = kk / KPL, ta = kk % KPL;
+            double av = buf[row + 16 * qa].a[ta];
+            double bv = buf[col + 16 * qa].b[ta];
+            acc += av * bv;
+        }
+        out[v] = acc;
+    }
+    emu::sync_wave();
+    return out;
+}
+inline emu_d4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, emu_d4 c, int, int, int) {
+    return emu_mfma16<double, emu_d4, 1>(&a, &b, c);
+}
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, emu_f4 c, int, int, int) {
+    return emu_mfma16<float, emu_f4, 1>(&a, &b, c);
+}
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf8 a, emu_bf8 b, emu_f4 c, int, int, int) {
+    __bf16 aa[8], bb[8];
+    for (int t = 0; t < 8; ++t) {
+        aa[t] = a[t];
+        bb[t] = b[t];
+    }
+    return emu_mfma16<__bf16, emu_f4, 8>(aa, bb, c);
+}
+
+// ---- runtime API ----------------------------------------------------------------------------
+hipError_t hipGetLastError();
+const char *hipGetErrorString(hipError_t);
+hipError_t hipMalloc(void **p, size_t n);
+template <typename T>
+inline hipError_t hipMalloc(T **p, size_t n) {
+    return hipMalloc((void **)p, n);
+}
+hipError_t hipGetDevice(int *d);
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int d);
+hipError_t hipFuncSetAttribute(const void *f, int attr, int value);
+#define hipDeviceAttributeMultiprocessorCount 63
+hipError_t hipDeviceGetAttribute(int *v, int attr, int dev);
+enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
+typedef void *hipEvent_t;
+hipError_t hipFree(void *p);
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k);
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t st);
+hipError_t hipStreamSynchronize(hipStream_t s);
+hipError_t hipDeviceSynchronize();
+hipError_t hipEventCreate(hipEvent_t *e);
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
+hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipEventDestroy(hipEvent_t e);
+
+namespace emu {
+void run_grid(dim3 grid, dim3 block, size_t shm, void (*body)(void *), void *ctx);
+}
+
+template <typename... KArgs, typename... Args>
+inline void hipLaunchKernelGGL(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shm, hipStream_t, Args... args) {
+    struct Ctx {
+        void (*k)(KArgs...);
+        std::tuple<KArgs...> a;
+    } ctx{k, std::tuple<KArgs...>((KArgs)args...)};
+    emu::run_grid(grid, block, shm, [](void *p) {
+        Ctx *c = (Ctx *)p;
+        std::apply(c->k, c->a);
+    }, &ctx);
+}
