@@ -1,0 +1,155 @@
+#!/usr/bin/env python
+"""TRG coarse-graining throughput on one MI355X: mixed precision vs FP64.
+
+Workload (leg dim 64, chi 64): a seeded Grassmann-even 4-leg site tensor (entries N(0,1) times
+exp(-0.5 * sum of leg popcounts), odd total parity zeroed, max|T| = 1, parity-preserving legs,
+statistics (1,1,-1,-1)) coarse-grained by Levin-Nave TRG for warmup + steps = 20 steps.  A step is
+one gtn.trg call (two signed permutations, two parity-block SVDs, six contractions, the norm).
+
+Prints one JSON line: value = steps/s of the mixed-precision path (contractions in split BF16 with
+FP32 accumulation, parity-block SVD / normalisation / log-norm sum in FP64), with the FP64 path's
+steps/s, ln Z per site of both against the FP64 CPU oracle, the roofline of the dominant kernel (the
+T' contraction GEMM) and the CPU baseline (the numpy oracle timed for one step on the host).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+D, CHI, SEED = 64, 64, 1
+MFMA_PEAK_TF = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
+
+
+def site_tensor(d, seed):
+    """The synthetic workload's site tensor on the host (numpy)."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((d, d, d, d))
+    i = np.arange(d)
+    pc = lambda x: np.array([bin(int(v)).count("1") for v in x])
+    enc = i ^ (pc(i >> 1) & 1)
+    w = np.exp(-0.5 * pc(enc))
+    par = i & 1
+    mask = (par[:, None, None, None] ^ par[None, :, None, None] ^ par[None, None, :, None] ^ par[None, None, None, :]) == 0
+    T = X * w[:, None, None, None] * w[None, :, None, None] * w[None, None, :, None] * w[None, None, None, :] * mask
+    return T / np.abs(T).max()
+
+
+def run_path(gtn, prec, svd_prec, steps, warmup, T0):
+    from grassmanntn_amd._device import Event, synchronize
+    from grassmanntn_amd.core import GemmProfile
+    T = gtn.dense(T0, encoder="parity-preserving", statistics=(1, 1, -1, -1))
+    logn = 0.0
+    lnz = []
+    with gtn.precision(prec, svd_prec):
+        for i in range(warmup):
+            T, nrm = gtn.trg(T, CHI)
+            logn = 2 * logn + math.log(nrm)
+        synchronize()
+        e0, e1 = Event(), Event()
+        with GemmProfile() as prof:
+            t0 = time.perf_counter()
+            e0.record()
+            for i in range(warmup, warmup + steps):
+                T, nrm = gtn.trg(T, CHI)
+                logn = 2 * logn + math.log(nrm)
+            e1.record()
+            synchronize()
+            wall = time.perf_counter() - t0
+        gpu_ms = e0.elapsed_ms(e1)
+    F = (gtn.logZ(T) + logn) / 2 ** (warmup + steps)
+    # T' contraction: two parity-block GEMMs of (chi^2/2) x (D^2/2) x (chi^2/2)
+    key = [k for k in prof.ms if k[0] == CHI * CHI // 2 and k[2] == D * D // 2 and k[3] == 2]
+    tp = prof.ms[key[0]] if key else []
+    return {"wall_s": wall, "gpu_ms": gpu_ms, "lnz": complex(F), "tprime_ms": tp}
+
+
+def cpu_baseline():
+    """One oracle TRG step at leg dim 64, chi 64 on the host (numpy + BLAS, all cores)."""
+    from oracle import gtn_oracle as O
+    T = O.site_tensor(D, SEED)
+    t0 = time.perf_counter()
+    O.trg(T, CHI)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "TRG steps/s", "cores": os.cpu_count(), "kind": "port",
+            "sample": "1 TRG step of the oracle (numpy FP64, per-parity-sector BLAS contractions, LAPACK "
+                      "block SVD) at leg dim 64, chi 64; %.1f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=18)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="bf16x3", help="contraction arithmetic of the mixed path")
+    ap.add_argument("--svd-precision", default="f32", help="range-finder arithmetic of the mixed path")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build runs the TRG hot path on a single GPU "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import grassmanntn_amd as gtn
+    T0 = site_tensor(D, SEED)
+    mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0)
+    fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "trg_golden.json")
+    n_total = args.warmup + args.steps
+    if os.path.exists(gpath):
+        g = json.load(open(gpath)).get("d64_chi64")
+        if g and n_total <= len(g["lnz_per_site_real"]):
+            golden = complex(g["lnz_per_site_real"][n_total - 1], g["lnz_per_site_imag"][n_total - 1])
+    sps = args.steps / (mixed["gpu_ms"] / 1000.0)
+    tp = mixed["tprime_ms"]
+    flop = 2 * 2.0 * (CHI * CHI // 2) * (D * D // 2) * (CHI * CHI // 2)
+    avg_ms = sum(tp) / len(tp) if tp else float("nan")
+    achieved = flop / (avg_ms / 1000.0) / 1e12 if tp else None
+    peak = MFMA_PEAK_TF[args.precision]
+    out = {
+        "metric": "TRG steps/s at chi=64 (mixed precision), leg dim 64",
+        "value": sps,
+        "unit": "TRG steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["gpu_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "%s contractions, %s range finder, f64 SVD/normalisation" % (args.precision, args.svd_precision),
+        "data": "synthetic",
+        "config": {"workload": "TRG, leg dim 64, chi 64, %d steps, seed %d" % (n_total, SEED),
+                   "leg_dim": D, "chi": CHI, "steps_total": n_total},
+        "lnz_per_site_mixed": [mixed["lnz"].real, mixed["lnz"].imag],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": (achieved / peak) if achieved else None, "traffic": None,
+                     "kernel": "T' contraction, 2 parity-block GEMMs of 2048^3, %s" % args.precision,
+                     "flop_per_launch": flop, "avg_launch_ms": avg_ms, "launches": len(tp)},
+    }
+    if fp64 is not None:
+        out["fp64_steps_per_s"] = args.steps / (fp64["gpu_ms"] / 1000.0)
+        out["speedup_mixed_vs_fp64"] = out["value"] / out["fp64_steps_per_s"]
+        out["lnz_per_site_fp64"] = [fp64["lnz"].real, fp64["lnz"].imag]
+        out["dlnz_mixed_vs_fp64_gpu"] = abs(mixed["lnz"] - fp64["lnz"])
+        if fp64["tprime_ms"]:
+            ms64 = sum(fp64["tprime_ms"]) / len(fp64["tprime_ms"])
+            out["fp64_tprime_tflops"] = flop / (ms64 / 1000.0) / 1e12
+    if golden is not None:
+        out["dlnz_mixed_vs_oracle"] = abs(mixed["lnz"] - golden)
+        if fp64 is not None:
+            out["dlnz_fp64_vs_oracle"] = abs(fp64["lnz"] - golden)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

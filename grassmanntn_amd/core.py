@@ -458,10 +458,43 @@ def _einsum1(plan, A, first, ignore):
     return res
 
 
+class GemmProfile:
+    """Collects HIP-event durations of every grouped-GEMM launch while active (bench only):
+    key (M, N, K, groups, batch, precision) -> list of milliseconds."""
+
+    active = None
+
+    def __init__(self):
+        self.pending = []
+        self.ms = {}
+
+    def __enter__(self):
+        GemmProfile.active = self
+        return self
+
+    def __exit__(self, *a):
+        GemmProfile.active = None
+        self.resolve()
+
+    def resolve(self):
+        for key, e0, e1 in self.pending:
+            self.ms.setdefault(key, []).append(e0.elapsed_ms(e1))
+        self.pending = []
+
+
 def _gemm(groups, dtype_code, prec):
     lib = L.lib()
     arr = (L.GemmDesc * len(groups))(*groups)
+    prof = GemmProfile.active
+    if prof is not None:
+        from ._device import Event
+        e0, e1 = Event(), Event()
+        e0.record()
     L.check(lib.gtn_gemm_grouped(arr, len(groups), dtype_code, L.PREC[prec], L.stream_ptr()), "gtn_gemm_grouped")
+    if prof is not None:
+        e1.record()
+        g = groups[0]
+        prof.pending.append(((int(g.M), int(g.N), int(g.K), len(groups), int(g.batch), prec), e0, e1))
 
 
 def _gemm_desc(A, sam, sak, B, sbk, sbn, Cp, scm, scn, M, N, K, batch=1, sab=0, sbb=0, scb=0, alpha=1.0, beta=0.0):
