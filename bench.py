@@ -82,9 +82,27 @@ def cpu_baseline():
                       "block SVD) at leg dim 64, chi 64; %.1f s" % dt}
 
 
+def copy_calibration(mib, reps=5):
+    """Run the library's 2-D copy kernel over `mib` MiB of FP64 `reps` times (reads and writes `mib` MiB
+    per launch): the known-size reference for scaling HBM byte counters."""
+    from grassmanntn_amd._device import DeviceArray, copy2d, synchronize
+    n = mib * (1 << 20) // 8
+    cols = 1 << 12
+    rows = n // cols
+    src = DeviceArray.zeros((rows, cols))
+    dst = DeviceArray.empty((rows, cols))
+    for _ in range(reps):
+        copy2d(dst, cols, src, cols, rows, cols)
+    synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": rows * cols * 8,
+                                           "bytes_written_per_launch": rows * cols * 8, "launches": reps}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run the copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--steps", type=int, default=18)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="bf16x3", help="contraction arithmetic of the mixed path")
@@ -97,6 +115,9 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import grassmanntn_amd as gtn
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
+        return
     T0 = site_tensor(D, SEED)
     mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0)
     fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
