@@ -757,17 +757,28 @@ def _interleave(AE, AO):
     return A
 
 
-def block_decompose(M, cutoff=None, kind="svd", block_fn=None):
+def block_decompose(M, cutoff=None, kind="svd", block_fn=None, rule="dense", kept=None):
     """BlockSVD / BlockEig: per-parity-block decomposition, int(cutoff/2) kept per block, both blocks
     padded to the next power of two and interleaved (__init__.py:3953-4029, 4347-4422).
 
+    rule="block" keeps ceil(cutoff/2) even and floor(cutoff/2) odd values, the truncation of
+    decompose_block (__init__.py:5076-5085); the block format records max(kept) live values per
+    parity (__init__.py:5092-5100) and the storage here is still padded to a power of two.
+    ``kept`` (a list) receives that max(kept).
     ``block_fn`` may replace the per-block LAPACK call (used to apply a gauge convention)."""
     ME, MO = parity_blocks(M)
-    half = None if cutoff is None else int(cutoff / 2)
+    if cutoff is None:
+        cE = cO = None
+    elif rule == "block":
+        cE, cO = int(math.ceil(cutoff / 2)), int(math.floor(cutoff / 2))
+    else:
+        cE = cO = int(cutoff / 2)
     fn = block_fn or (sorted_svd if kind == "svd" else sorted_eig)
-    UE, sE, VE = fn(ME, half)
-    UO, sO, VO = fn(MO, half)
+    UE, sE, VE = fn(ME, cE)
+    UO, sO, VO = fn(MO, cO)
     d = max(len(sE), len(sO))
+    if kept is not None:
+        kept.append(d)
     d = int(2 ** math.ceil(math.log2(d))) if d > 0 else 1
 
     def pad(U, s, V):
@@ -791,7 +802,7 @@ def _parse_partition(string, name):
     return string
 
 
-def _decompose(T, string, cutoff, kind, block_fn=None):
+def _decompose(T, string, cutoff, kind, block_fn=None, rule="dense", kept=None):
     """Shared body of svd (__init__.py:4033-4308) and eig (__init__.py:4425-4698)."""
     name = kind
     string = _parse_partition(string, name)
@@ -847,7 +858,7 @@ def _decompose(T, string, cutoff, kind, block_fn=None):
         U, s, V = fn(O.data, cutoff)
         S = np.diag(s)
     else:
-        U, S, V = block_decompose(O.data, cutoff, kind, block_fn)
+        U, S, V = block_decompose(O.data, cutoff, kind, block_fn, rule, kept)
     ls, rs = -1, 1
     kw = dict(encoder="parity-preserving", format="matrix", skip_power_of_two_check=True)
     if O.statistics[0] == 0:
