@@ -866,10 +866,12 @@ def split_legs(T, string_inp, final_stat, final_shape, intermediate_stat=None, s
 # --------------------------------------------------------------------------------------------------
 # svd / eig (__init__.py:3931-4698)
 # --------------------------------------------------------------------------------------------------
-def _block_decompose_device(Mdata, cutoff, mode):
+def _block_decompose_device(Mdata, cutoff, mode, rule="dense", kept=None):
     """Parity blocks of a parity-preserving (-1,+1) matrix, decomposed on the device.
     Returns (U, S, Vt) full interleaved matrices exactly as BlockSVD / BlockEig lay them out
-    (__init__.py:3953-4029, 4347-4422): int(cutoff/2) per block, padded to a power of two."""
+    (__init__.py:3953-4029, 4347-4422): int(cutoff/2) per block, padded to a power of two.
+    rule="block" keeps ceil(cutoff/2) even and floor(cutoff/2) odd values (decompose_block,
+    __init__.py:5076-5085); `kept` (a list) receives the larger kept count."""
     m, n = Mdata.shape
     if not _pkg_flag("skip_parity_blocking_check", False):
         leak = _parity_leak(Mdata)
@@ -877,18 +879,25 @@ def _block_decompose_device(Mdata, cutoff, mode):
             fail("Error[BlockSVD]: This matrix is not constructed from a Grassmann-even tensor.")
     ME = _strided_block(Mdata, 0)
     MO = _strided_block(Mdata, 1)
-    half = None if cutoff is None else int(cutoff / 2)
+    if cutoff is None:
+        caps = (None, None)
+    elif rule == "block":
+        caps = (int(math.ceil(cutoff / 2)), int(math.floor(cutoff / 2)))
+    else:
+        caps = (int(cutoff / 2), int(cutoff / 2))
     blocks = []
-    for Mb in (ME, MO):
+    for Mb, cap in zip((ME, MO), caps):
         mb, nb_ = Mb.shape
         k = min(mb, nb_)
-        if half is not None:
-            k = min(k, half)
+        if cap is not None:
+            k = min(k, cap)
         blocks.append((Mb, k))
     outs = _svd_blocks_raw(blocks, mode)
     # rank rule: count nonzero singular values per block (k values read back)
     kk = [int(np.count_nonzero(o[1].numpy())) if o[3] > 0 else 0 for o in outs]
     d = max(kk)
+    if kept is not None:
+        kept.append(d)
     d = int(2 ** math.ceil(math.log2(d))) if d > 0 else 1
     mh, nh = m // 2, n // 2
     dt = Mdata.dtype
@@ -982,7 +991,7 @@ def _svd_blocks_raw(blocks, mode):
     return outs
 
 
-def _decompose(T, string, cutoff, mode):
+def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
     name = "svd" if mode == 0 else "eig"
     if not isinstance(T, dense):
         fail("Error[%s]: Object type must only be dense or sparse!" % name)
@@ -1036,7 +1045,7 @@ def _decompose(T, string, cutoff, mode):
             copy2d(Vt, nn, V_b, nn, nz, nn)
             copy2d(S, nz + 1, S_b, 1, nz, 1)
     else:
-        U, S, Vt = _block_decompose_device(Md, cutoff, mode)
+        U, S, Vt = _block_decompose_device(Md, cutoff, mode, rule, kept)
     ls, rs = -1, 1
     s0, s1 = M.statistics
 

@@ -8,17 +8,23 @@ from .core import fail, FERMI
 
 
 def zcap(T):
-    """Trace the two bosonic flavour legs with an all-ones capper (gauge2d.py:1591-1597)."""
+    """Trace the two bosonic flavour legs with an all-ones capper (gauge2d.py:1591-1597,
+    gauge2d_block.py:1590-1599)."""
+    from .block import block
     n = T.shape[4]
     capper = gtn.dense(np.full((n, n), 1.0), statistics=(0, 0))
+    if isinstance(T, block):
+        capper = capper.toblock()
     return gtn.einsum("IJKLij,ij->IJKL", T, capper)
 
 
 def logZ(T, boundary_conditions="periodic"):
     """ln of the supertrace einsum('IJIJ') (gauge2d.py:1599-1615).  The anti-periodic sign uses the
     popcount of the index in T's own encoder, as the reference does.  A negative real Z gives the
-    complex logarithm (the reference's np.log returns nan there)."""
-    if boundary_conditions == "anti-periodic":
+    complex logarithm (the reference's np.log returns nan there).  Block tensors get no anti-periodic
+    signs, as in the block driver (gauge2d_block.py:1601-1615)."""
+    from .block import block
+    if boundary_conditions == "anti-periodic" and not isinstance(T, block):
         T = T.copy()
         # sign (-1)^popcount of the stored index along leg 1, as a signed gather
         from ._perm import Perm, strides
