@@ -9,7 +9,7 @@ the HIP kernels of libgtn_hip.so (C ABI: include/gtn_hip.h); there is no CPU fal
 from . import param
 from ._lib import GrassmannDeviceError
 from ._plan import GrassmannError, denumerate
-from .core import (dense, sparse, einsum, join_legs, split_legs, svd, eig, hconjugate, power, sqrt,
+from .core import (dense, sparse, einsum, join_legs, split_legs, svd, eig, svd_many, hconjugate, power, sqrt,
                    is_grassmann_even, precision, settings)
 from .block import block, zero_block, zero_block_eo, einsum_block, svd_block, eig_block, hconjugate_block
 from .gauge2d import zcap, logZ, trg, atrg2dy, atrg2dx

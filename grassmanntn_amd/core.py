@@ -866,19 +866,16 @@ def split_legs(T, string_inp, final_stat, final_shape, intermediate_stat=None, s
 # --------------------------------------------------------------------------------------------------
 # svd / eig (__init__.py:3931-4698)
 # --------------------------------------------------------------------------------------------------
-def _block_decompose_device(Mdata, cutoff, mode, rule="dense", kept=None):
-    """Parity blocks of a parity-preserving (-1,+1) matrix, decomposed on the device.
-    Returns (U, S, Vt) full interleaved matrices exactly as BlockSVD / BlockEig lay them out
-    (__init__.py:3953-4029, 4347-4422): int(cutoff/2) per block, padded to a power of two.
-    rule="block" keeps ceil(cutoff/2) even and floor(cutoff/2) odd values (decompose_block,
-    __init__.py:5076-5085); `kept` (a list) receives the larger kept count."""
+def _block_prepare(Mdata, cutoff, rule="dense"):
+    """Parity blocks ME, MO of a parity-preserving (-1,+1) matrix and the number of values to compute
+    in each: int(cutoff/2) per block (BlockSVD / BlockEig, __init__.py:3953-4029, 4347-4422), or
+    ceil(cutoff/2) even and floor(cutoff/2) odd with rule="block" (decompose_block,
+    __init__.py:5076-5085)."""
     m, n = Mdata.shape
     if not _pkg_flag("skip_parity_blocking_check", False):
         leak = _parity_leak(Mdata)
         if leak / (m * n / 2) > 1.0e-14:
             fail("Error[BlockSVD]: This matrix is not constructed from a Grassmann-even tensor.")
-    ME = _strided_block(Mdata, 0)
-    MO = _strided_block(Mdata, 1)
     if cutoff is None:
         caps = (None, None)
     elif rule == "block":
@@ -886,21 +883,26 @@ def _block_decompose_device(Mdata, cutoff, mode, rule="dense", kept=None):
     else:
         caps = (int(cutoff / 2), int(cutoff / 2))
     blocks = []
-    for Mb, cap in zip((ME, MO), caps):
+    for par, cap in enumerate(caps):
+        Mb = _strided_block(Mdata, par)
         mb, nb_ = Mb.shape
         k = min(mb, nb_)
         if cap is not None:
             k = min(k, cap)
         blocks.append((Mb, k))
-    outs = _svd_blocks_raw(blocks, mode)
-    # rank rule: count nonzero singular values per block (k values read back)
+    return blocks
+
+
+def _block_assemble(outs, m, n, dt, kept=None):
+    """Full interleaved (U, S, Vt) from the two blocks' results, both padded to the next power of two
+    of the larger kept count (the rank rule of SortedSVD counts the nonzero values read back);
+    `kept` (a list) receives that count."""
     kk = [int(np.count_nonzero(o[1].numpy())) if o[3] > 0 else 0 for o in outs]
     d = max(kk)
     if kept is not None:
         kept.append(d)
     d = int(2 ** math.ceil(math.log2(d))) if d > 0 else 1
     mh, nh = m // 2, n // 2
-    dt = Mdata.dtype
     # stack the kept columns / rows of both blocks, zero-padded to d, then interleave by parity
     SU = DeviceArray.zeros((2, mh, d), dt)
     SV = DeviceArray.zeros((2, d, nh), dt)
@@ -991,7 +993,12 @@ def _svd_blocks_raw(blocks, mode):
     return outs
 
 
-def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
+class _Job:
+    """One svd / eig call in flight: the joined matrix, its blocks and what the split needs."""
+    __slots__ = ("T", "left", "right", "cutoff", "M", "blocks", "blocked", "kept", "done")
+
+
+def _decompose_prepare(T, string, cutoff, mode, rule):
     name = "svd" if mode == 0 else "eig"
     if not isinstance(T, dense):
         fail("Error[%s]: Object type must only be dense or sparse!" % name)
@@ -1000,7 +1007,10 @@ def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
     for st in T.statistics:
         if st not in ALLOWED_STAT:
             fail("Error[%s]: The input object contains illegal statistics. (0, 1, -1, or * only)" % name)
+    job = _Job()
+    job.T, job.cutoff, job.kept, job.done = T, cutoff, [], None
     left, right = parse_partition(string, name)
+    job.left, job.right = left, right
     fmt, enc = T.format, T.encoder
     xstats, xshape = T.statistics, T.shape
     nl = len(left)
@@ -1027,15 +1037,35 @@ def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
             out = [x.switch_format() for x in out]
         if enc == "parity-preserving":
             out = [x.switch_encoder() for x in out]
-        return tuple(out)
+        job.done = tuple(out)
+        job.blocks = []
+        return job
     inter = (0 if all(s == 0 for s in sl) else -1, 0 if all(s == 0 for s in sr) else 1)
     M = join_legs(T, "(" + left + ")(" + right + ")", "matrix", intermediate_stat=inter)
     M = M.force_encoder("parity-preserving")
+    job.M = M
     Md = M.data
     if M.statistics[0] == 0 or M.statistics[1] == 0:
         mm, nn = Md.shape
         k = min(mm, nn) if cutoff is None else min(mm, nn, cutoff)
-        U_b, S_b, V_b, k = _svd_blocks_raw([(Md, k)], mode)[0]
+        job.blocked = False
+        job.blocks = [(Md, k)]
+    else:
+        job.blocked = True
+        job.blocks = _block_prepare(Md, cutoff, rule)
+    return job
+
+
+def _decompose_finish(job, outs):
+    T, M = job.T, job.M
+    Md = M.data
+    fmt, enc = T.format, T.encoder
+    xstats, xshape = T.statistics, T.shape
+    left, right = job.left, job.right
+    nl = len(left)
+    if not job.blocked:
+        mm, nn = Md.shape
+        U_b, S_b, V_b, k = outs[0]
         nz = int(np.count_nonzero(S_b.numpy())) if k else 0
         U = DeviceArray.zeros((mm, max(nz, 1)), np.float64)
         Vt = DeviceArray.zeros((max(nz, 1), nn), np.float64)
@@ -1045,7 +1075,7 @@ def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
             copy2d(Vt, nn, V_b, nn, nz, nn)
             copy2d(S, nz + 1, S_b, 1, nz, 1)
     else:
-        U, S, Vt = _block_decompose_device(Md, cutoff, mode, rule, kept)
+        U, S, Vt = _block_assemble(outs, Md.shape[0], Md.shape[1], Md.dtype, job.kept)
     ls, rs = -1, 1
     s0, s1 = M.statistics
 
@@ -1079,6 +1109,34 @@ def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
     for x in out:
         x._even = True
     return tuple(out)
+
+
+def _decompose_many(calls, mode, rule="dense"):
+    """Several svd / eig calls [(T, string, cutoff), ...] with all their parity blocks decomposed in
+    one gtn_svd_blocks call.  Returns [(U, S, V, kept), ...] (kept: the larger per-parity count)."""
+    jobs = [_decompose_prepare(T, string, cutoff, mode, rule) for T, string, cutoff in calls]
+    allb = [b for j in jobs for b in j.blocks]
+    outs = _svd_blocks_raw(allb, mode) if allb else []
+    res, at = [], 0
+    for j in jobs:
+        nb = len(j.blocks)
+        r = j.done if j.done is not None else _decompose_finish(j, outs[at:at + nb])
+        at += nb
+        res.append(tuple(r) + (j.kept[0] if j.kept else None,))
+    return res
+
+
+def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
+    U, S, V, k = _decompose_many([(T, string, cutoff)], mode, rule)[0]
+    if kept is not None and k is not None:
+        kept.append(k)
+    return U, S, V
+
+
+def svd_many(calls, mode=0):
+    """[(T, string, cutoff), ...] -> [(U, S, V), ...]: several dense svd calls (eig with mode=1) whose
+    parity blocks share one batched device decomposition."""
+    return [r[:3] for r in _decompose_many(calls, mode)]
 
 
 def svd(InpObj, string, cutoff=None, save_memory=False):

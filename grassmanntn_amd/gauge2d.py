@@ -54,8 +54,12 @@ def trg(T, dcut=64, iternum=None, error_test=False):
         fail("Error[trg]: The statistics must be (1,1,-1,-1)!")
     T1 = gtn.einsum("ijkl->jkli", T)
     T2 = gtn.einsum("ijkl->klij", T)
-    U1, S1, V1 = T1.svd("ab|cd", dcut)
-    U2, S2, V2 = T2.svd("ab|cd", dcut)
+    if isinstance(T1, gtn.dense):
+        # both decompositions' parity blocks in one batched device call
+        (U1, S1, V1), (U2, S2, V2) = gtn.svd_many([(T1, "ab|cd", dcut), (T2, "ab|cd", dcut)])
+    else:
+        U1, S1, V1 = T1.svd("ab|cd", dcut)
+        U2, S2, V2 = T2.svd("ab|cd", dcut)
     r = gtn.sqrt(S1)
     U1 = gtn.einsum("abx,xc->abc", U1, r)
     V1 = gtn.einsum("ax,xbc->abc", r, V1)
@@ -82,8 +86,12 @@ def atrg2dy(T1, T2, dcut=64, intermediate_dcut=None, iternum=None, error_test=Fa
         intermediate_dcut = dcut
     T1 = gtn.einsum("ijkl->li jk", T1)
     T2 = gtn.einsum("ijkl->li jk", T2)
-    U1, S1, V1 = T1.svd("li|jk", intermediate_dcut)
-    U2, S2, V2 = T2.svd("li|jk", intermediate_dcut)
+    if isinstance(T1, gtn.dense):
+        (U1, S1, V1), (U2, S2, V2) = gtn.svd_many([(T1, "li|jk", intermediate_dcut),
+                                                    (T2, "li|jk", intermediate_dcut)])
+    else:
+        U1, S1, V1 = T1.svd("li|jk", intermediate_dcut)
+        U2, S2, V2 = T2.svd("li|jk", intermediate_dcut)
     A = V1.copy()
     B = gtn.einsum("lia,ab->lib", U1, S1)
     Cc = gtn.einsum("ab,bjk->ajk", S2, V2)
