@@ -98,6 +98,7 @@ def _bind(lib):
         "gtn_permute": (C.c_int, [C.POINTER(PermDesc), vp, vp, vp]),
         "gtn_sign_tensor": (C.c_int, [i32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), vp, vp]),
         "gtn_gemm_grouped": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),
+        "gtn_mfma_selftest": (C.c_int, [vp, vp]),
         "gtn_sumsq": (C.c_int, [vp, i64, i32, vp, vp]),
         "gtn_scale_by_norm": (C.c_int, [vp, i64, i32, vp, vp]),
         "gtn_reduce_last": (C.c_int, [vp, vp, i64, i64, i32, vp]),

@@ -882,22 +882,19 @@ def _block_prepare(Mdata, cutoff, rule="dense"):
         caps = (int(math.ceil(cutoff / 2)), int(math.floor(cutoff / 2)))
     else:
         caps = (int(cutoff / 2), int(cutoff / 2))
-    blocks = []
+    specs = []
     for par, cap in enumerate(caps):
-        Mb = _strided_block(Mdata, par)
-        mb, nb_ = Mb.shape
-        k = min(mb, nb_)
+        k = min(m // 2, n // 2)
         if cap is not None:
             k = min(k, cap)
-        blocks.append((Mb, k))
-    return blocks
+        specs.append((Mdata, par, k))
+    return specs
 
 
-def _block_assemble(outs, m, n, dt, kept=None):
+def _block_assemble(outs, kk, m, n, dt, kept=None):
     """Full interleaved (U, S, Vt) from the two blocks' results, both padded to the next power of two
-    of the larger kept count (the rank rule of SortedSVD counts the nonzero values read back);
+    of the larger kept count kk (the rank rule of SortedSVD: the nonzero values read back);
     `kept` (a list) receives that count."""
-    kk = [int(np.count_nonzero(o[1].numpy())) if o[3] > 0 else 0 for o in outs]
     d = max(kk)
     if kept is not None:
         kept.append(d)
@@ -948,8 +945,9 @@ def _parity_leak(M):
     return math.sqrt(float(out.item()))
 
 
-def _strided_block(M, par):
-    """Contiguous copy of M[par::2, par::2] (the ME / MO extraction of __init__.py:3990-3996)."""
+def _strided_block(M, par, out=None):
+    """Contiguous copy of M[par::2, par::2] (the ME / MO extraction of __init__.py:3990-3996), into
+    `out` when given."""
     m, n = M.shape
     P = Perm()
     i0 = P.add_in(max(m - 1, 1), n)
@@ -961,26 +959,75 @@ def _strided_block(M, par):
     P.add_out(m // 2, [(v0, 1)])
     P.add_out(n // 2, [(v1, 1)])
     off = par * n + par
-    return P.run(M.flat(off, M.numel() - off))
+    return P.run(M.flat(off, M.numel() - off), dst=out)
+
+
+def _runs(keys):
+    """Split a list into maximal runs of equal consecutive keys: [(start, count), ...]."""
+    out = []
+    for i, k in enumerate(keys):
+        if out and keys[out[-1][0]] == k:
+            out[-1][1] += 1
+        else:
+            out.append([i, 1])
+    return out
+
+
+def _materialize(specs):
+    """specs: [(matrix, parity or None, k)] -> [(contiguous matrix, k)].  Parity blocks of equal shape
+    that follow each other are gathered into one buffer, so that the device decomposition sees them at
+    a constant stride and batches them."""
+    keys = [(None if par is None else (M.shape[0] // 2, M.shape[1] // 2), i if par is None else -1)
+            for i, (M, par, k) in enumerate(specs)]
+    res = []
+    for st, cnt in _runs(keys):
+        M0, par0, _ = specs[st]
+        if par0 is None:
+            res.append((M0, specs[st][2]))
+            continue
+        mh, nh = M0.shape[0] // 2, M0.shape[1] // 2
+        buf = DeviceArray.empty((cnt, mh, nh), M0.dtype)
+        for j in range(cnt):
+            M, par, k = specs[st + j]
+            view = buf.flat(j * mh * nh, mh * nh, (mh, nh))
+            _strided_block(M, par, out=view)
+            res.append((view, k))
+    return res
 
 
 def _svd_blocks_raw(blocks, mode):
-    """blocks: list of (contiguous FP64 matrix, k).  Returns per block (U (m x k), S (k), Vt (k x n), k)."""
+    """blocks: list of (contiguous FP64 matrix, k).  Returns (per block (U (m x k), S (k), Vt (k x n), k),
+    per block the number of nonzero values in S).  Consecutive blocks of equal shape get their outputs
+    in shared buffers at constant strides."""
     lib = L.lib()
     nblk = len(blocks)
     arr = (L.SvdBlock * nblk)()
-    outs = []
-    for i, (Mb, k) in enumerate(blocks):
+    outs = [None] * nblk
+    counts = [0] * nblk
+    keys = []
+    for Mb, k in blocks:
         mb, nb_ = Mb.shape
-        kk = k if (mb and nb_) else 0
-        U = DeviceArray.zeros((mb, max(kk, 1)), np.float64)
-        S = DeviceArray.zeros((max(kk, 1),), np.float64)
-        V = DeviceArray.zeros((max(kk, 1), nb_), np.float64)
-        outs.append((U, S, V, kk))
-        arr[i].M = Mb.data_ptr() if Mb.numel() else None
-        arr[i].U, arr[i].S, arr[i].Vt = U.data_ptr(), S.data_ptr(), V.data_ptr()
-        arr[i].m, arr[i].n, arr[i].lda, arr[i].ldu, arr[i].ldv = mb, nb_, max(nb_, 1), max(kk, 1), max(nb_, 1)
-        arr[i].k = kk
+        keys.append((mb, nb_, k if (mb and nb_) else 0))
+    groups = _runs(keys)
+    sbufs = []
+    for st, cnt in groups:
+        mb, nb_, kk = keys[st]
+        k1 = max(kk, 1)
+        U = DeviceArray.zeros((cnt, mb, k1), np.float64)
+        S = DeviceArray.zeros((cnt, k1), np.float64)
+        V = DeviceArray.zeros((cnt, k1, nb_), np.float64)
+        sbufs.append(S)
+        for j in range(cnt):
+            i = st + j
+            Mb = blocks[i][0]
+            u = U.flat(j * mb * k1, mb * k1, (mb, k1))
+            sv = S.flat(j * k1, k1, (k1,))
+            v = V.flat(j * k1 * nb_, k1 * nb_, (k1, nb_))
+            outs[i] = (u, sv, v, kk)
+            arr[i].M = Mb.data_ptr() if Mb.numel() else None
+            arr[i].U, arr[i].S, arr[i].Vt = u.data_ptr(), sv.data_ptr(), v.data_ptr()
+            arr[i].m, arr[i].n, arr[i].lda, arr[i].ldu, arr[i].ldv = mb, nb_, max(nb_, 1), k1, max(nb_, 1)
+            arr[i].k = kk
     opts = L.SvdOpts()
     opts.oversample = settings.svd_oversample
     opts.iters = settings.svd_iters
@@ -990,7 +1037,16 @@ def _svd_blocks_raw(blocks, mode):
     ws_bytes = lib.gtn_svd_workspace_bytes(arr, nblk, C.byref(opts))
     ws = DeviceArray.empty(max(ws_bytes // 8, 1), np.float64)
     L.check(lib.gtn_svd_blocks(arr, nblk, C.byref(opts), L.ptr(ws), ws_bytes, L.stream_ptr()), "gtn_svd_blocks")
-    return outs
+    # the rank rule's count of nonzero values, one read-back per group
+    for (st, cnt), S in zip(groups, sbufs):
+        kk = keys[st][2]
+        h = S.numpy()
+        if not np.all(np.isfinite(h)):
+            raise L.GrassmannDeviceError("gtn_svd_blocks returned non-finite singular values (input holds "
+                                         "inf/nan or the decomposition broke down)")
+        for j in range(cnt):
+            counts[st + j] = int(np.count_nonzero(h[j][:kk])) if kk else 0
+    return outs, counts
 
 
 class _Job:
@@ -1049,14 +1105,14 @@ def _decompose_prepare(T, string, cutoff, mode, rule):
         mm, nn = Md.shape
         k = min(mm, nn) if cutoff is None else min(mm, nn, cutoff)
         job.blocked = False
-        job.blocks = [(Md, k)]
+        job.blocks = [(Md, None, k)]
     else:
         job.blocked = True
         job.blocks = _block_prepare(Md, cutoff, rule)
     return job
 
 
-def _decompose_finish(job, outs):
+def _decompose_finish(job, outs, counts):
     T, M = job.T, job.M
     Md = M.data
     fmt, enc = T.format, T.encoder
@@ -1066,7 +1122,7 @@ def _decompose_finish(job, outs):
     if not job.blocked:
         mm, nn = Md.shape
         U_b, S_b, V_b, k = outs[0]
-        nz = int(np.count_nonzero(S_b.numpy())) if k else 0
+        nz = counts[0]
         U = DeviceArray.zeros((mm, max(nz, 1)), np.float64)
         Vt = DeviceArray.zeros((max(nz, 1), nn), np.float64)
         S = DeviceArray.zeros((max(nz, 1), max(nz, 1)), np.float64)
@@ -1075,7 +1131,7 @@ def _decompose_finish(job, outs):
             copy2d(Vt, nn, V_b, nn, nz, nn)
             copy2d(S, nz + 1, S_b, 1, nz, 1)
     else:
-        U, S, Vt = _block_assemble(outs, Md.shape[0], Md.shape[1], Md.dtype, job.kept)
+        U, S, Vt = _block_assemble(outs, counts, Md.shape[0], Md.shape[1], Md.dtype, job.kept)
     ls, rs = -1, 1
     s0, s1 = M.statistics
 
@@ -1115,12 +1171,12 @@ def _decompose_many(calls, mode, rule="dense"):
     """Several svd / eig calls [(T, string, cutoff), ...] with all their parity blocks decomposed in
     one gtn_svd_blocks call.  Returns [(U, S, V, kept), ...] (kept: the larger per-parity count)."""
     jobs = [_decompose_prepare(T, string, cutoff, mode, rule) for T, string, cutoff in calls]
-    allb = [b for j in jobs for b in j.blocks]
-    outs = _svd_blocks_raw(allb, mode) if allb else []
+    specs = [b for j in jobs for b in j.blocks]
+    outs, counts = _svd_blocks_raw(_materialize(specs), mode) if specs else ([], [])
     res, at = [], 0
     for j in jobs:
         nb = len(j.blocks)
-        r = j.done if j.done is not None else _decompose_finish(j, outs[at:at + nb])
+        r = j.done if j.done is not None else _decompose_finish(j, outs[at:at + nb], counts[at:at + nb])
         at += nb
         res.append(tuple(r) + (j.kept[0] if j.kept else None,))
     return res

@@ -295,6 +295,36 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
             }
 }
 
+// Lane-layout self-test: A[i][0] = 1, A[i][1] = i, B[0][j] = j + 1, B[1][j] = 16 (other k zero), so
+// D[i][j] = 16 i + j + 1.  Each lane feeds the operand elements the GEMM kernels assume it holds and
+// stores its four result registers.  One wavefront; writes exactly 768 doubles.
+__device__ __forceinline__ double probe_a(int i, int k) { return k == 0 ? 1.0 : (k == 1 ? (double)i : 0.0); }
+__device__ __forceinline__ double probe_b(int k, int j) { return k == 0 ? (double)(j + 1) : (k == 1 ? 16.0 : 0.0); }
+
+__global__ void __launch_bounds__(64) k_mfma_selftest(double *out) {
+    const int l = threadIdx.x;
+    if (l >= 64) return;
+    const int r = l % 16, q = l / 16;
+    d4 a64 = {0, 0, 0, 0};
+    a64 = __builtin_amdgcn_mfma_f64_16x16x4f64(probe_a(r, q), probe_b(q, r), a64, 0, 0, 0);
+    f4 a32 = {0, 0, 0, 0};
+    a32 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)probe_a(r, q), (float)probe_b(q, r), a32, 0, 0, 0);
+    bf8 av, bv;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        av[t] = (__bf16)(float)probe_a(r, 8 * q + t);
+        bv[t] = (__bf16)(float)probe_b(8 * q + t, r);
+    }
+    f4 ab = {0, 0, 0, 0};
+    ab = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, ab, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        out[4 * l + v] = a64[v];
+        out[256 + 4 * l + v] = (double)a32[v];
+        out[512 + 4 * l + v] = (double)ab[v];
+    }
+}
+
 template <typename TIN>
 static int launch_gemm(GemmArgs &a, int precision, hipStream_t s) {
     int total = a.tile_start[a.n_groups];
@@ -393,5 +423,12 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
         int rc = dtype == GTN_F64 ? launch_gemm<double>(a, precision, s) : launch_gemm<float>(a, precision, s);
         if (rc) return rc;
     }
+    return GTN_OK;
+}
+
+extern "C" int gtn_mfma_selftest(double *out_dev, void *stream) {
+    GTN_REQUIRE(out_dev, "gtn_mfma_selftest: null output");
+    hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, as_stream(stream), out_dev);
+    GTN_CHECK_LAUNCH();
     return GTN_OK;
 }

@@ -6,8 +6,11 @@
 // Rayleigh-Ritz step whose p x p symmetric eigenproblem is solved by cyclic Jacobi in LDS.
 // Small blocks (min(m, n) <= kPMax): the Gram matrix of the short side (or the Hermitian block
 // itself in eig mode) is diagonalised directly by the same Jacobi kernel.
-// Every stage processes all blocks of the call together: GEMMs are grouped (one group per block)
-// and the LDS kernels run one workgroup per block.
+// Every stage processes all blocks of the call together.  Consecutive blocks of equal shape whose
+// matrices and workspaces sit at constant strides form a "run": each stage issues one batched launch
+// per run (batched GEMM groups, blockIdx.y over the run for the elementwise kernels, one workgroup
+// per block for the LDS kernels), so a call over many equal blocks costs the same number of launches
+// as a call over one.
 #include "gtn_common.h"
 #include <string.h>
 #include <math.h>
@@ -39,7 +42,9 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
-__global__ void k_fill_gaussian(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed) {
+__global__ void k_fill_gaussian(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, int64_t dx) {
+    x += blockIdx.y * dx;
+    seed = mix64(seed + blockIdx.y);
     int64_t n = rows * cols;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t h1 = mix64(seed ^ mix64((uint64_t)i * 2 + 1));
@@ -52,7 +57,9 @@ __global__ void k_fill_gaussian(double *x, int64_t rows, int64_t cols, int64_t l
     }
 }
 
-__global__ void k_sum_partials(const double *part, int64_t count, int S, double *out) {
+__global__ void k_sum_partials(const double *part, int64_t count, int S, double *out, int64_t dpart, int64_t dout) {
+    part += blockIdx.y * dpart;
+    out += blockIdx.y * dout;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
         double s = 0;
         for (int b = 0; b < S; ++b) s += part[b * count + i];
@@ -60,7 +67,10 @@ __global__ void k_sum_partials(const double *part, int64_t count, int S, double 
     }
 }
 
-__global__ void k_copy2d(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows, int64_t cols) {
+__global__ void k_copy2d(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows, int64_t cols,
+                         int64_t dsrc, int64_t ddst) {
+    src += blockIdx.y * dsrc;
+    dst += blockIdx.y * ddst;
     int64_t total = rows * cols;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = e / cols, c = e - r * cols;
@@ -68,14 +78,18 @@ __global__ void k_copy2d(const double *src, int64_t lds, double *dst, int64_t ld
     }
 }
 // dst (cols x rows) = src (rows x cols)^T
-__global__ void k_transpose2d(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows, int64_t cols) {
+__global__ void k_transpose2d(const double *src, int64_t lds, double *dst, int64_t ldd, int64_t rows, int64_t cols,
+                              int64_t dsrc, int64_t ddst) {
+    src += blockIdx.y * dsrc;
+    dst += blockIdx.y * ddst;
     int64_t total = rows * cols;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t c = e / rows, r = e - c * rows;
         dst[c * ldd + r] = src[r * lds + c];
     }
 }
-__global__ void k_fill(double *x, int64_t rows, int64_t cols, int64_t ld, double v) {
+__global__ void k_fill(double *x, int64_t rows, int64_t cols, int64_t ld, double v, int64_t dx) {
+    x += blockIdx.y * dx;
     int64_t total = rows * cols;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = e / cols, c = e - r * cols;
@@ -86,17 +100,44 @@ __global__ void k_fill(double *x, int64_t rows, int64_t cols, int64_t ld, double
 // ---------------------------------------------------------------------------------------------
 // LDS kernels, one workgroup per block
 // ---------------------------------------------------------------------------------------------
+// Either up to kMaxBatch independent blocks (uniform == 0: pointers and sizes per entry), or any
+// number of equal blocks at constant strides (uniform == 1: entry 0 plus the strides d_*).
 struct SmallBatch {
     int n;
+    int uniform;
     int p[kMaxBatch];
     const double *in[kMaxBatch];
     double *out[kMaxBatch];
     double *aux[kMaxBatch];
     double *lam[kMaxBatch];
+    int64_t d_in, d_out, d_aux, d_lam;
     double shift_rel;
     int by_abs;
     int max_sweeps;
 };
+struct SmallPtrs {
+    int p;
+    const double *in;
+    double *out, *aux, *lam;
+};
+__device__ __forceinline__ SmallPtrs small_ptrs(const SmallBatch &B) {
+    SmallPtrs r;
+    const int64_t bi = blockIdx.x;
+    if (B.uniform) {
+        r.p = B.p[0];
+        r.in = B.in[0] + bi * B.d_in;
+        r.out = B.out[0] + bi * B.d_out;
+        r.aux = B.aux[0] ? B.aux[0] + bi * B.d_aux : nullptr;
+        r.lam = B.lam[0] ? B.lam[0] + bi * B.d_lam : nullptr;
+    } else {
+        r.p = B.p[bi];
+        r.in = B.in[bi];
+        r.out = B.out[bi];
+        r.aux = B.aux[bi];
+        r.lam = B.lam[bi];
+    }
+    return r;
+}
 
 __device__ double wg_sum(double v, double *sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -118,10 +159,10 @@ __device__ __forceinline__ int up_idx(int i, int j, int p) { return i * p - (i *
 __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
     __shared__ double red[16];
-    const int bi = blockIdx.x;
-    const int p = B.p[bi];
-    const double *W = B.in[bi];
-    double *Rinv = B.out[bi];
+    const SmallPtrs P = small_ptrs(B);
+    const int p = P.p;
+    const double *W = P.in;
+    double *Rinv = P.out;
     const int np = p * (p + 1) / 2;
     double *A = sm;       // packed upper R
     double *X = sm + np;  // packed upper R^{-1}
@@ -175,12 +216,12 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
 __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
     __shared__ double red[16];
-    const int bi = blockIdx.x;
-    const int p = B.p[bi];
-    const double *Cin = B.in[bi];
-    double *V = B.aux[bi];
-    double *Vout = B.out[bi];
-    double *lam = B.lam[bi];
+    const SmallPtrs P = small_ptrs(B);
+    const int p = P.p;
+    const double *Cin = P.in;
+    double *V = P.aux;
+    double *Vout = P.out;
+    double *lam = P.lam;
     double *A = sm;
     double *cs = sm + p * p;
     int *pr = (int *)(cs + p);
@@ -249,13 +290,17 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
             __syncthreads();
         }
     }
+    // sort keys: lambda (or |lambda|), with NaN ranked below everything, ties broken by index, so
+    // that the ranks are a permutation of 0..p-1 whatever the values
     for (int e = tid; e < p; e += nt) {
         double ke = A[e * p + e];
         if (B.by_abs) ke = fabs(ke);
+        if (ke != ke) ke = -INFINITY;
         int rank = 0;
         for (int f = 0; f < p; ++f) {
             double kf = A[f * p + f];
             if (B.by_abs) kf = fabs(kf);
+            if (kf != kf) kf = -INFINITY;
             if (kf > ke || (kf == ke && f < e)) ++rank;
         }
         order[rank] = e;
@@ -273,24 +318,34 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
 // ---------------------------------------------------------------------------------------------
 // sigma_i = sqrt(max(lam_i, 0)) (svd) or lam_i (eig); the rank rule |s_i| / (|s_0| + 1e-14) > 1e-14
 // of __init__.py:3939-3941 gives keep_i in {0, 1}; inv_i = keep_i / |s_i|.
-__global__ void k_sigma(const double *lam, int k, int eig_mode, double *S, double *inv, double *keep) {
+__global__ void k_sigma(const double *lam, int k, int eig_mode, double *S, double *inv, double *keep, int64_t dlam,
+                        int64_t dS, int64_t dw) {
+    lam += blockIdx.y * dlam;
+    S += blockIdx.y * dS;
+    inv += blockIdx.y * dw;
+    keep += blockIdx.y * dw;
     int i = threadIdx.x + blockIdx.x * blockDim.x;
     if (i >= k) return;
     double s0 = eig_mode ? fabs(lam[0]) : sqrt(fmax(lam[0], 0.0));
     double s = eig_mode ? lam[i] : sqrt(fmax(lam[i], 0.0));
     bool kp = fabs(s / (s0 + 1e-14)) > 1e-14;
-    S[i] = kp ? s : 0.0;
+    // a NaN eigenvalue is passed on, so that the caller sees the breakdown
+    S[i] = (lam[i] != lam[i]) ? lam[i] : (kp ? s : 0.0);
     inv[i] = kp ? 1.0 / fabs(s) : 0.0;
     keep[i] = kp ? 1.0 : 0.0;
 }
-__global__ void k_scale_rows(double *X, int k, int64_t n, int64_t ld, const double *f) {
+__global__ void k_scale_rows(double *X, int k, int64_t n, int64_t ld, const double *f, int64_t dX, int64_t df) {
+    X += blockIdx.y * dX;
+    f += blockIdx.y * df;
     int64_t total = (int64_t)k * n;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = e / n, c = e - r * n;
         X[r * ld + c] *= f[r];
     }
 }
-__global__ void k_scale_cols(double *X, int64_t m, int k, int64_t ld, const double *f) {
+__global__ void k_scale_cols(double *X, int64_t m, int k, int64_t ld, const double *f, int64_t dX, int64_t df) {
+    X += blockIdx.y * dX;
+    f += blockIdx.y * df;
     int64_t total = m * k;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = e / k, c = e - r * k;
@@ -299,8 +354,10 @@ __global__ void k_scale_cols(double *X, int64_t m, int k, int64_t ld, const doub
 }
 // gauge: column j of U gets its largest-|.| entry (first in row order) positive; the matching row
 // of Vt (when given) flips with it.  One workgroup per column.
-__global__ void k_gauge(double *U, int64_t m, int64_t ldu, double *Vt, int64_t n, int64_t ldv) {
+__global__ void k_gauge(double *U, int64_t m, int64_t ldu, double *Vt, int64_t n, int64_t ldv, int64_t dU, int64_t dV) {
     const int j = blockIdx.x;
+    U += blockIdx.y * dU;
+    if (Vt) Vt += blockIdx.y * dV;
     __shared__ double bv[256];
     __shared__ int64_t bi[256];
     double best = -1;
@@ -430,21 +487,178 @@ static int run(std::vector<gtn_gemm_desc> &g, int prec, hipStream_t s) {
     return gtn_gemm_grouped(g.data(), (int)g.size(), GTN_F64, prec, (void *)s);
 }
 
-// launch an LDS kernel over the blocks, kMaxBatch at a time
-template <typename F>
-static int small_launch(std::vector<Blk *> &bs, F fill, void (*kern)(SmallBatch), size_t shm_per_p2, size_t shm_extra,
-                        SmallBatch proto, hipStream_t s) {
-    for (size_t base = 0; base < bs.size(); base += kMaxBatch) {
-        SmallBatch sb = proto;
-        sb.n = 0;
-        int64_t pmax = 0;
-        for (size_t i = base; i < bs.size() && i < base + kMaxBatch; ++i) {
-            fill(*bs[i], sb, sb.n);
-            if (bs[i]->p > pmax) pmax = bs[i]->p;
-            ++sb.n;
+// A run: consecutive blocks of equal geometry whose input / output matrices and workspaces advance
+// by constant strides (in doubles), so that every stage can process the run in one launch.
+struct Run {
+    Blk *f;          // first block
+    int n;           // blocks in the run (the batch of every launch)
+    int first;       // index of the first block in the call (seeds)
+    int64_t dM, dU, dS, dV, dws;
+};
+
+static bool delta_of(const void *a, const void *b, int64_t &d) {
+    if ((a == nullptr) != (b == nullptr)) return false;
+    if (!a) {
+        d = 0;
+        return true;
+    }
+    int64_t diff = (int64_t)((const char *)b - (const char *)a);
+    if (diff % 8) return false;
+    d = diff / 8;
+    return true;
+}
+
+static std::vector<Run> make_runs(std::vector<Blk *> &bs, std::vector<int> &index) {
+    std::vector<Run> runs;
+    for (size_t i = 0; i < bs.size(); ++i) {
+        Blk *b = bs[i];
+        if (!runs.empty()) {
+            Run &r = runs.back();
+            Blk *a = r.f;
+            const gtn_svd_block &x = a->b, &y = b->b;
+            bool same = x.m == y.m && x.n == y.n && x.k == y.k && x.lda == y.lda && x.ldu == y.ldu &&
+                        x.ldv == y.ldv && a->p == b->p && a->small == b->small && a->left == b->left &&
+                        a->bytes == b->bytes && r.n < 65535;
+            int64_t dM, dU, dS, dV, dws;
+            // the previous block of the run is f + (n-1) strides; compare with the last one
+            Blk *last = bs[i - 1];
+            if (same && delta_of(last->b.M, y.M, dM) && delta_of(last->b.U, y.U, dU) && delta_of(last->b.S, y.S, dS) &&
+                delta_of(last->b.Vt, y.Vt, dV) && delta_of(last->ws, b->ws, dws)) {
+                if (r.n == 1) {
+                    r.dM = dM, r.dU = dU, r.dS = dS, r.dV = dV, r.dws = dws;
+                    ++r.n;
+                    continue;
+                }
+                if (dM == r.dM && dU == r.dU && dS == r.dS && dV == r.dV && dws == r.dws) {
+                    ++r.n;
+                    continue;
+                }
+            }
         }
+        Run r;
+        r.f = b;
+        r.n = 1;
+        r.first = index[i];
+        r.dM = r.dU = r.dS = r.dV = r.dws = 0;
+        runs.push_back(r);
+    }
+    return runs;
+}
+
+// range checks (dry run) of a strided family of n ranges
+#define R8N(p, elems, delta, n, what)                                                     \
+    do {                                                                                    \
+        R8((p), (elems), (what));                                                           \
+        if ((n) > 1) R8((p) + (int64_t)((n) - 1) * (delta), (elems), (what));               \
+    } while (0)
+
+static dim3 grid2(int64_t elems, int n) { return dim3(grid_for(elems, 256, 4096), n); }
+
+// gemm over a run: C_b = A_b B_b with the per-block strides of the run
+static gtn_gemm_desc gdr(const double *A, int64_t sam, int64_t sak, const double *B, int64_t sbk, int64_t sbn, double *C,
+                         int64_t scm, int64_t M, int64_t N, int64_t K, int n, int64_t dA, int64_t dB, int64_t dC) {
+    return gd(A, sam, sak, B, sbk, sbn, C, scm, M, N, K, n, dA, dB, dC);
+}
+
+// split of a Gram reduction of length K: none for long runs (enough workgroups already), at most 8
+// chunks for short runs (one grouped launch), up to 32 for a single block
+static int split_of(const Run &r, int64_t K) {
+    if (r.n >= 64) return 1;
+    int S = pick_split(K);
+    if (r.n > 1 && S > 8) S = 8;
+    return S;
+}
+
+// descriptors of W_b = A_b A_b^T for every block b of a run, where A (p x K) has element (i, kk) at
+// A[i * sam + kk * sak] and consecutive K-chunks start chunk_stride * (K / S) apart; with S > 1 the
+// chunks' products go to the partials buffer `part` (S consecutive p x p matrices)
+static void gram_descs(std::vector<gtn_gemm_desc> &g, const Run &r, const double *A, int64_t sam, int64_t sak, int64_t K,
+                       int64_t chunk_stride, double *out, double *part, int64_t p, int S) {
+    const int64_t ch = K / S;
+    if (S == 1) {
+        g.push_back(gdr(A, sam, sak, A, sak, sam, out, p, p, p, K, r.n, r.dws, r.dws, r.dws));
+    } else if (r.n == 1) {
+        g.push_back(gd(A, sam, sak, A, sak, sam, part, p, p, p, ch, S, ch * chunk_stride, ch * chunk_stride, p * p));
+    } else {
+        for (int c = 0; c < S; ++c)
+            g.push_back(gdr(A + c * ch * chunk_stride, sam, sak, A + c * ch * chunk_stride, sak, sam, part + (int64_t)c * p * p,
+                            p, p, p, ch, r.n, r.dws, r.dws, r.dws));
+    }
+}
+
+static int sum_partials(std::vector<Run> &runs, int64_t (*K_of)(Blk &), double *(*out)(Blk &), hipStream_t s) {
+    for (Run &r : runs) {
+        Blk &f = *r.f;
+        const int64_t p = f.p;
+        const int S = split_of(r, K_of(f));
+        if (S == 1) continue;
+        R8N(f.part, (int64_t)S * p * p, r.dws, r.n, "gram partials");
+        R8N(out(f), p * p, r.dws, r.n, "gram");
+        LAUNCH(k_sum_partials, grid2(p * p, r.n), dim3(256), 0, s, (const double *)f.part, p * p, S, out(f), r.dws, r.dws);
+    }
+    return GTN_OK;
+}
+
+static int64_t rows_of(Blk &b) { return b.b.m; }
+static int64_t cols_of(Blk &b) { return b.b.n; }
+
+// Gram matrix W = Y^T Y (Y rows x p, ld p) of every block
+static int gram_Y(std::vector<Run> &runs, hipStream_t s) {
+    std::vector<gtn_gemm_desc> g;
+    for (Run &r : runs) {
+        Blk &f = *r.f;
+        const int64_t rows = f.b.m, p = f.p;
+        R8N(f.Y, rows * p, r.dws, r.n, "gram input");
+        gram_descs(g, r, f.Y, 1, p, rows, p, f.W, f.part, p, split_of(r, rows));
+    }
+    int rc = run(g, GTN_PREC_F64, s);
+    if (rc) return rc;
+    return sum_partials(runs, rows_of, [](Blk &b) { return b.W; }, s);
+}
+
+// launch an LDS kernel over the runs: one uniform launch per multi-block run; single blocks are
+// packed kMaxBatch to a launch
+template <typename F>
+static int small_launch(std::vector<Run> &runs, F fill, void (*kern)(SmallBatch), size_t shm_per_p2, size_t shm_extra,
+                        SmallBatch proto, hipStream_t s) {
+    std::vector<Run *> singles;
+    auto go = [&](SmallBatch &sb, int64_t pmax, int grid) -> int {
         size_t shm = shm_per_p2 * pmax * pmax + shm_extra * pmax;
         GTN_REQUIRE(shm + 256 <= 160 * 1024, "gtn_svd_blocks: LDS request of %zu bytes exceeds 160 KiB", shm);
+        if (dry_run()) return GTN_OK;
+        // LDS beyond 64 KiB must be requested explicitly (gfx950 has 160 KiB per CU)
+        GTN_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        LAUNCH(kern, dim3(grid), dim3(1024), shm, s, sb);
+        return GTN_OK;
+    };
+    for (Run &r : runs) {
+        if (r.n == 1) {
+            singles.push_back(&r);
+            continue;
+        }
+        SmallBatch sb = proto;
+        sb.n = r.n;
+        sb.uniform = 1;
+        fill(*r.f, sb, 0);
+        sb.d_in = sb.d_out = sb.d_aux = sb.d_lam = r.dws;
+        const int64_t p2 = (int64_t)sb.p[0] * sb.p[0];
+        R8N(sb.in[0], p2, r.dws, r.n, "small kernel input");
+        R8N(sb.out[0], p2, r.dws, r.n, "small kernel output");
+        if (sb.aux[0]) R8N(sb.aux[0], p2, r.dws, r.n, "small kernel aux");
+        if (sb.lam[0]) R8N(sb.lam[0], sb.p[0], r.dws, r.n, "small kernel lambda");
+        int rc = go(sb, sb.p[0], r.n);
+        if (rc) return rc;
+    }
+    for (size_t base = 0; base < singles.size(); base += kMaxBatch) {
+        SmallBatch sb = proto;
+        sb.n = 0;
+        sb.uniform = 0;
+        int64_t pmax = 0;
+        for (size_t i = base; i < singles.size() && i < base + kMaxBatch; ++i) {
+            fill(*singles[i]->f, sb, sb.n);
+            if (singles[i]->f->p > pmax) pmax = singles[i]->f->p;
+            ++sb.n;
+        }
         for (int i = 0; i < sb.n; ++i) {
             int64_t p2 = (int64_t)sb.p[i] * sb.p[i];
             R8(sb.in[i], p2, "small kernel input");
@@ -452,39 +666,22 @@ static int small_launch(std::vector<Blk *> &bs, F fill, void (*kern)(SmallBatch)
             if (sb.aux[i]) R8(sb.aux[i], p2, "small kernel aux");
             if (sb.lam[i]) R8(sb.lam[i], sb.p[i], "small kernel lambda");
         }
-        if (dry_run()) continue;
-        // LDS beyond 64 KiB must be requested explicitly (gfx950 has 160 KiB per CU)
-        GTN_CHECK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        LAUNCH(kern, dim3(sb.n), dim3(1024), shm, s, sb);
+        int rc = go(sb, pmax, sb.n);
+        if (rc) return rc;
     }
     return GTN_OK;
 }
 
-// orthonormalise X (rows x p, ld p) of every block in place (CholeskyQR, `passes` times)
-static int orthonormalise(std::vector<Blk *> &bs, double *(*pick)(Blk &), bool rows_m, int passes, hipStream_t s) {
+// orthonormalise Y (rows x p, ld p) of every block in place (CholeskyQR, `passes` times)
+static int orthonormalise(std::vector<Run> &runs, int passes, hipStream_t s) {
     for (int pass = 0; pass < passes; ++pass) {
-        std::vector<gtn_gemm_desc> g;
-        for (Blk *b : bs) {
-            double *X = pick(*b);
-            int64_t rows = rows_m ? b->b.m : b->b.n, p = b->p;
-            int S = pick_split(rows);
-            int64_t ch = rows / S;
-            g.push_back(gd(X, 1, p, X, p, 1, b->part, p, p, p, ch, S, ch * p, ch * p, p * p));
-        }
-        int rc = run(g, GTN_PREC_F64, s);
+        int rc = gram_Y(runs, s);
         if (rc) return rc;
-        for (Blk *b : bs) {
-            int64_t rows = rows_m ? b->b.m : b->b.n;
-            R8(b->part, (int64_t)pick_split(rows) * b->p * b->p, "gram partials");
-            R8(b->W, b->p * b->p, "gram");
-            LAUNCH(k_sum_partials, dim3(grid_for(b->p * b->p, 256)), dim3(256), 0, s, (const double *)b->part,
-                               b->p * b->p, pick_split(rows), b->W);
-        }
         SmallBatch proto;
         memset(&proto, 0, sizeof(proto));
         proto.shift_rel = pass == 0 ? 1e-13 : 0.0;
         rc = small_launch(
-            bs,
+            runs,
             [](Blk &b, SmallBatch &sb, int i) {
                 sb.p[i] = (int)b.p;
                 sb.in[i] = b.W;
@@ -492,31 +689,32 @@ static int orthonormalise(std::vector<Blk *> &bs, double *(*pick)(Blk &), bool r
             },
             k_chol_inv, sizeof(double), sizeof(double), proto, s);
         if (rc) return rc;
-        g.clear();
-        for (Blk *b : bs) {
-            int64_t rows = rows_m ? b->b.m : b->b.n, p = b->p;
-            g.push_back(gd(pick(*b), p, 1, b->R, p, 1, b->T, p, rows, p, p));
+        std::vector<gtn_gemm_desc> g;
+        for (Run &r : runs) {
+            Blk &f = *r.f;
+            int64_t rows = f.b.m, p = f.p;
+            g.push_back(gdr(f.Y, p, 1, f.R, p, 1, f.T, p, rows, p, p, r.n, r.dws, r.dws, r.dws));
         }
         rc = run(g, GTN_PREC_F64, s);
         if (rc) return rc;
-        for (Blk *b : bs) {
-            int64_t rows = rows_m ? b->b.m : b->b.n, p = b->p;
-            R8(b->T, rows * p, "orth scratch");
-            R8(pick(*b), rows * p, "orth basis");
-            LAUNCH(k_copy2d, dim3(grid_for(rows * p, 256)), dim3(256), 0, s, (const double *)b->T, p, pick(*b), p,
-                               rows, p);
+        for (Run &r : runs) {
+            Blk &f = *r.f;
+            int64_t rows = f.b.m, p = f.p;
+            R8N(f.T, rows * p, r.dws, r.n, "orth scratch");
+            R8N(f.Y, rows * p, r.dws, r.n, "orth basis");
+            LAUNCH(k_copy2d, grid2(rows * p, r.n), dim3(256), 0, s, (const double *)f.T, p, f.Y, p, rows, p, r.dws, r.dws);
         }
     }
     return GTN_OK;
 }
 
-static int jacobi(std::vector<Blk *> &bs, int by_abs, hipStream_t s) {
+static int jacobi(std::vector<Run> &runs, int by_abs, hipStream_t s) {
     SmallBatch proto;
     memset(&proto, 0, sizeof(proto));
     proto.by_abs = by_abs;
     proto.max_sweeps = 30;
     return small_launch(
-        bs,
+        runs,
         [](Blk &b, SmallBatch &sb, int i) {
             sb.p[i] = (int)b.p;
             sb.in[i] = b.G;
@@ -530,172 +728,197 @@ static int jacobi(std::vector<Blk *> &bs, int by_abs, hipStream_t s) {
 static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream_t s) {
     const int eig = o.mode == 1;
     std::vector<Blk *> small, large, all;
-    for (Blk &b : blocks) {
+    std::vector<int> ismall, ilarge, iall;
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        Blk &b = blocks[i];
         if (b.b.k <= 0 || b.b.m <= 0 || b.b.n <= 0) continue;
         all.push_back(&b);
-        (b.small ? small : large).push_back(&b);
+        iall.push_back((int)i);
+        if (b.small) {
+            small.push_back(&b);
+            ismall.push_back((int)i);
+        } else {
+            large.push_back(&b);
+            ilarge.push_back((int)i);
+        }
     }
     if (all.empty()) return GTN_OK;
+    std::vector<Run> rs = make_runs(small, ismall), rl = make_runs(large, ilarge), ra = make_runs(all, iall);
     int rc;
     // ---- small blocks: G = Gram of the short side (svd) or the block itself (eig)
-    if (!small.empty()) {
+    if (!rs.empty()) {
         std::vector<gtn_gemm_desc> g;
-        for (Blk *b : small) {
-            const gtn_svd_block &B = b->b;
-            R8(b->G, b->p * b->p, "small gram");
-            LAUNCH(k_fill, dim3(grid_for(b->p * b->p, 256)), dim3(256), 0, s, b->G, b->p, b->p, b->p, 0.0);
+        for (Run &r : rs) {
+            Blk &f = *r.f;
+            const gtn_svd_block &B = f.b;
+            R8N(f.G, f.p * f.p, r.dws, r.n, "small gram");
+            LAUNCH(k_fill, grid2(f.p * f.p, r.n), dim3(256), 0, s, f.G, f.p, f.p, f.p, 0.0, r.dws);
             if (eig) {
-                GTN_REQUIRE(B.m <= b->p && B.n <= b->p, "gtn_svd_blocks: eig block larger than its work size");
-                R8(B.M, ext(B.m, B.n, B.lda), "small eig input");
-                LAUNCH(k_copy2d, dim3(grid_for(B.m * B.n, 256)), dim3(256), 0, s, B.M, B.lda, b->G, b->p, B.m, B.n);
-            } else if (b->left) {
-                g.push_back(gd(B.M, B.lda, 1, B.M, 1, B.lda, b->G, b->p, B.m, B.m, B.n));  // M M^T
+                GTN_REQUIRE(B.m <= f.p && B.n <= f.p, "gtn_svd_blocks: eig block larger than its work size");
+                R8N(B.M, ext(B.m, B.n, B.lda), r.dM, r.n, "small eig input");
+                LAUNCH(k_copy2d, grid2(B.m * B.n, r.n), dim3(256), 0, s, B.M, B.lda, f.G, f.p, B.m, B.n, r.dM, r.dws);
+            } else if (f.left) {
+                g.push_back(gdr(B.M, B.lda, 1, B.M, 1, B.lda, f.G, f.p, B.m, B.m, B.n, r.n, r.dM, r.dM, r.dws));  // M M^T
             } else {
-                g.push_back(gd(B.M, 1, B.lda, B.M, B.lda, 1, b->G, b->p, B.n, B.n, B.m));  // M^T M
+                g.push_back(gdr(B.M, 1, B.lda, B.M, B.lda, 1, f.G, f.p, B.n, B.n, B.m, r.n, r.dM, r.dM, r.dws));  // M^T M
             }
         }
         rc = run(g, GTN_PREC_F64, s);
         if (rc) return rc;
     }
     // ---- large blocks: randomized subspace iteration
-    if (!large.empty()) {
+    if (!rl.empty()) {
         std::vector<gtn_gemm_desc> g;
-        int bi = 0;
-        for (Blk *b : large) {
-            R8(b->Z, b->b.n * b->p, "gaussian");
-            LAUNCH(k_fill_gaussian, dim3(grid_for(b->b.n * b->p, 256)), dim3(256), 0, s, b->Z, b->b.n, b->p, b->p,
-                               o.seed * 0x9E3779B97F4A7C15ull + (uint64_t)(bi++));
-            g.push_back(gd(b->b.M, b->b.lda, 1, b->Z, b->p, 1, b->Y, b->p, b->b.m, b->p, b->b.n));  // Y = M Z
+        for (Run &r : rl) {
+            Blk &f = *r.f;
+            R8N(f.Z, f.b.n * f.p, r.dws, r.n, "gaussian");
+            R8N(f.b.M, ext(f.b.m, f.b.n, f.b.lda), r.dM, r.n, "block");
+            LAUNCH(k_fill_gaussian, grid2(f.b.n * f.p, r.n), dim3(256), 0, s, f.Z, f.b.n, f.p, f.p,
+                   o.seed * 0x9E3779B97F4A7C15ull + (uint64_t)r.first, r.dws);
+            g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));  // Y = M Z
         }
         rc = run(g, o.precision, s);
         if (rc) return rc;
-        auto pickY = [](Blk &b) { return b.Y; };
-        rc = orthonormalise(large, pickY, true, o.iters > 0 ? 1 : 2, s);
+        rc = orthonormalise(rl, o.iters > 0 ? 1 : 2, s);
         if (rc) return rc;
         for (int it = 0; it < o.iters; ++it) {
             g.clear();
-            for (Blk *b : large) {
+            for (Run &r : rl) {
+                Blk &f = *r.f;
                 if (eig)  // Z = M Y
-                    g.push_back(gd(b->b.M, b->b.lda, 1, b->Y, b->p, 1, b->Z, b->p, b->b.m, b->p, b->b.n));
+                    g.push_back(gdr(f.b.M, f.b.lda, 1, f.Y, f.p, 1, f.Z, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));
                 else  // Z = M^T Y
-                    g.push_back(gd(b->b.M, 1, b->b.lda, b->Y, b->p, 1, b->Z, b->p, b->b.n, b->p, b->b.m));
+                    g.push_back(gdr(f.b.M, 1, f.b.lda, f.Y, f.p, 1, f.Z, f.p, f.b.n, f.p, f.b.m, r.n, r.dM, r.dws, r.dws));
             }
             rc = run(g, o.precision, s);
             if (rc) return rc;
             if (!eig) {
                 g.clear();
-                for (Blk *b : large)  // Y = M Z
-                    g.push_back(gd(b->b.M, b->b.lda, 1, b->Z, b->p, 1, b->Y, b->p, b->b.m, b->p, b->b.n));
+                for (Run &r : rl) {  // Y = M Z
+                    Blk &f = *r.f;
+                    g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));
+                }
                 rc = run(g, o.precision, s);
                 if (rc) return rc;
             } else {
-                for (Blk *b : large) {
-                    R8(b->Z, b->b.m * b->p, "eig iterate");
-                    R8(b->Y, b->b.m * b->p, "eig basis");
-                    LAUNCH(k_copy2d, dim3(grid_for(b->b.m * b->p, 256)), dim3(256), 0, s, (const double *)b->Z,
-                                       b->p, b->Y, b->p, b->b.m, b->p);
+                for (Run &r : rl) {
+                    Blk &f = *r.f;
+                    R8N(f.Z, f.b.m * f.p, r.dws, r.n, "eig iterate");
+                    R8N(f.Y, f.b.m * f.p, r.dws, r.n, "eig basis");
+                    LAUNCH(k_copy2d, grid2(f.b.m * f.p, r.n), dim3(256), 0, s, (const double *)f.Z, f.p, f.Y, f.p,
+                           f.b.m, f.p, r.dws, r.dws);
                 }
             }
-            rc = orthonormalise(large, pickY, true, it + 1 == o.iters ? 2 : 1, s);
+            rc = orthonormalise(rl, it + 1 == o.iters ? 2 : 1, s);
             if (rc) return rc;
         }
         // Rayleigh-Ritz in FP64: Bm = Y^T M (p x n); G = Bm Bm^T (svd) or Bm Y (eig)
         g.clear();
-        for (Blk *b : large)
-            g.push_back(gd(b->Y, 1, b->p, b->b.M, b->b.lda, 1, b->Bm, b->b.n, b->p, b->b.n, b->b.m));
-        rc = run(g, GTN_PREC_F64, s);
-        if (rc) return rc;
-        g.clear();
-        for (Blk *b : large) {
-            if (eig) {
-                g.push_back(gd(b->Bm, b->b.n, 1, b->Y, b->p, 1, b->G, b->p, b->p, b->p, b->b.n));
-            } else {
-                int S = pick_split(b->b.n);
-                int64_t ch = b->b.n / S;
-                g.push_back(gd(b->Bm, b->b.n, 1, b->Bm, 1, b->b.n, b->part, b->p, b->p, b->p, ch, S, ch, ch, b->p * b->p));
-            }
+        for (Run &r : rl) {
+            Blk &f = *r.f;
+            g.push_back(gdr(f.Y, 1, f.p, f.b.M, f.b.lda, 1, f.Bm, f.b.n, f.p, f.b.n, f.b.m, r.n, r.dws, r.dM, r.dws));
         }
         rc = run(g, GTN_PREC_F64, s);
         if (rc) return rc;
-        if (!eig) {
-            for (Blk *b : large) {
-                R8(b->part, (int64_t)pick_split(b->b.n) * b->p * b->p, "ritz partials");
-                R8(b->G, b->p * b->p, "ritz gram");
-                LAUNCH(k_sum_partials, dim3(grid_for(b->p * b->p, 256)), dim3(256), 0, s, (const double *)b->part,
-                                   b->p * b->p, pick_split(b->b.n), b->G);
+        if (eig) {
+            g.clear();
+            for (Run &r : rl) {
+                Blk &f = *r.f;
+                g.push_back(gdr(f.Bm, f.b.n, 1, f.Y, f.p, 1, f.G, f.p, f.p, f.p, f.b.n, r.n, r.dws, r.dws, r.dws));
             }
+            rc = run(g, GTN_PREC_F64, s);
+            if (rc) return rc;
+        } else {
+            // G = Bm Bm^T (Bm p x n, ld n)
+            g.clear();
+            for (Run &r : rl) {
+                Blk &f = *r.f;
+                R8N(f.Bm, f.p * f.b.n, r.dws, r.n, "ritz input");
+                gram_descs(g, r, f.Bm, f.b.n, 1, f.b.n, 1, f.G, f.part, f.p, split_of(r, f.b.n));
+            }
+            rc = run(g, GTN_PREC_F64, s);
+            if (rc) return rc;
+            rc = sum_partials(rl, cols_of, [](Blk &b) { return b.G; }, s);
+            if (rc) return rc;
         }
     }
     // ---- p x p eigenproblems of every block at once
-    rc = jacobi(all, eig, s);
+    rc = jacobi(ra, eig, s);
     if (rc) return rc;
-    for (Blk *b : all) {
-        GTN_REQUIRE(b->b.k <= 256 && b->b.k <= b->p, "gtn_svd_blocks: k too large");
-        R8(b->lam, b->b.k, "lambda");
-        R8(b->b.S, b->b.k, "S");
-        R8(b->inv, b->b.k, "inv");
-        R8(b->keep, b->b.k, "keep");
-        LAUNCH(k_sigma, dim3(1), dim3(256), 0, s, (const double *)b->lam, b->b.k, eig, b->b.S, b->inv, b->keep);
+    for (Run &r : ra) {
+        Blk &f = *r.f;
+        GTN_REQUIRE(f.b.k <= 256 && f.b.k <= f.p, "gtn_svd_blocks: k too large");
+        R8N(f.lam, f.b.k, r.dws, r.n, "lambda");
+        R8N(f.b.S, f.b.k, r.dS, r.n, "S");
+        R8N(f.inv, f.b.k, r.dws, r.n, "inv");
+        R8N(f.keep, f.b.k, r.dws, r.n, "keep");
+        LAUNCH(k_sigma, dim3(1, r.n), dim3(256), 0, s, (const double *)f.lam, f.b.k, eig, f.b.S, f.inv, f.keep, r.dws, r.dS,
+               r.dws);
     }
     // ---- singular vectors
     std::vector<gtn_gemm_desc> g;
-    for (Blk *b : all) {
-        const gtn_svd_block &B = b->b;
-        const int64_t k = B.k, p = b->p;
-        if (b->small) {
-            if (eig || b->left) {
+    for (Run &r : ra) {
+        Blk &f = *r.f;
+        const gtn_svd_block &B = f.b;
+        const int64_t k = B.k, p = f.p;
+        if (f.small) {
+            if (eig || f.left) {
                 GTN_REQUIRE(B.m <= p, "gtn_svd_blocks: small block rows exceed the work size");
-                R8(b->Vs, ext(B.m, k, p), "eigenvectors");
-                R8(B.U, ext(B.m, k, B.ldu), "U");
-                LAUNCH(k_copy2d, dim3(grid_for(B.m * k, 256)), dim3(256), 0, s, (const double *)b->Vs, p, B.U, B.ldu,
-                                   B.m, k);
-                if (!eig) g.push_back(gd(b->Vs, 1, p, B.M, B.lda, 1, B.Vt, B.ldv, k, B.n, B.m));  // Vt = U^T M
+                R8N(f.Vs, ext(B.m, k, p), r.dws, r.n, "eigenvectors");
+                R8N(B.U, ext(B.m, k, B.ldu), r.dU, r.n, "U");
+                LAUNCH(k_copy2d, grid2(B.m * k, r.n), dim3(256), 0, s, (const double *)f.Vs, p, B.U, B.ldu, B.m, k, r.dws,
+                       r.dU);
+                if (!eig)  // Vt = U^T M
+                    g.push_back(gdr(f.Vs, 1, p, B.M, B.lda, 1, B.Vt, B.ldv, k, B.n, B.m, r.n, r.dws, r.dM, r.dV));
             } else {
                 GTN_REQUIRE(B.n <= p, "gtn_svd_blocks: small block columns exceed the work size");
-                R8(b->Vs, ext(B.n, k, p), "eigenvectors");
-                R8(B.Vt, ext(k, B.n, B.ldv), "Vt");
-                LAUNCH(k_transpose2d, dim3(grid_for(B.n * k, 256)), dim3(256), 0, s, (const double *)b->Vs, p, B.Vt,
-                                   B.ldv, B.n, k);  // Vt = Vs[:, :k]^T
-                g.push_back(gd(B.M, B.lda, 1, b->Vs, p, 1, B.U, B.ldu, B.m, k, B.n));  // U = M V
+                R8N(f.Vs, ext(B.n, k, p), r.dws, r.n, "eigenvectors");
+                R8N(B.Vt, ext(k, B.n, B.ldv), r.dV, r.n, "Vt");
+                LAUNCH(k_transpose2d, grid2(B.n * k, r.n), dim3(256), 0, s, (const double *)f.Vs, p, B.Vt, B.ldv, B.n, k,
+                       r.dws, r.dV);  // Vt = Vs[:, :k]^T
+                g.push_back(gdr(B.M, B.lda, 1, f.Vs, p, 1, B.U, B.ldu, B.m, k, B.n, r.n, r.dM, r.dws, r.dU));  // U = M V
             }
         } else {
-            g.push_back(gd(b->Y, p, 1, b->Vs, p, 1, B.U, B.ldu, B.m, k, p));  // U = Y Vs
-            if (!eig) g.push_back(gd(b->Vs, 1, p, b->Bm, B.n, 1, B.Vt, B.ldv, k, B.n, p));  // Vt = Vs^T Bm
+            g.push_back(gdr(f.Y, p, 1, f.Vs, p, 1, B.U, B.ldu, B.m, k, p, r.n, r.dws, r.dws, r.dU));  // U = Y Vs
+            if (!eig)  // Vt = Vs^T Bm
+                g.push_back(gdr(f.Vs, 1, p, f.Bm, B.n, 1, B.Vt, B.ldv, k, B.n, p, r.n, r.dws, r.dws, r.dV));
         }
     }
     rc = run(g, GTN_PREC_F64, s);
     if (rc) return rc;
-    for (Blk *b : all) {
-        const gtn_svd_block &B = b->b;
+    for (Run &r : ra) {
+        Blk &f = *r.f;
+        const gtn_svd_block &B = f.b;
         const int64_t k = B.k;
         if (!eig) {
-            bool u_from_m = b->small && !b->left;
-            R8(B.U, ext(B.m, k, B.ldu), "U");
-            R8(B.Vt, ext(k, B.n, B.ldv), "Vt");
-            R8(b->inv, k, "inv");
-            R8(b->keep, k, "keep");
+            bool u_from_m = f.small && !f.left;
+            R8N(B.U, ext(B.m, k, B.ldu), r.dU, r.n, "U");
+            R8N(B.Vt, ext(k, B.n, B.ldv), r.dV, r.n, "Vt");
+            R8N(f.inv, k, r.dws, r.n, "inv");
+            R8N(f.keep, k, r.dws, r.n, "keep");
             if (u_from_m) {
-                LAUNCH(k_scale_cols, dim3(grid_for(B.m * k, 256)), dim3(256), 0, s, B.U, B.m, (int)k, B.ldu,
-                                   (const double *)b->inv);
-                LAUNCH(k_scale_rows, dim3(grid_for(k * B.n, 256)), dim3(256), 0, s, B.Vt, (int)k, B.n, B.ldv,
-                                   (const double *)b->keep);
+                LAUNCH(k_scale_cols, grid2(B.m * k, r.n), dim3(256), 0, s, B.U, B.m, (int)k, B.ldu, (const double *)f.inv,
+                       r.dU, r.dws);
+                LAUNCH(k_scale_rows, grid2(k * B.n, r.n), dim3(256), 0, s, B.Vt, (int)k, B.n, B.ldv,
+                       (const double *)f.keep, r.dV, r.dws);
             } else {
-                LAUNCH(k_scale_rows, dim3(grid_for(k * B.n, 256)), dim3(256), 0, s, B.Vt, (int)k, B.n, B.ldv,
-                                   (const double *)b->inv);
-                LAUNCH(k_scale_cols, dim3(grid_for(B.m * k, 256)), dim3(256), 0, s, B.U, B.m, (int)k, B.ldu,
-                                   (const double *)b->keep);
+                LAUNCH(k_scale_rows, grid2(k * B.n, r.n), dim3(256), 0, s, B.Vt, (int)k, B.n, B.ldv, (const double *)f.inv,
+                       r.dV, r.dws);
+                LAUNCH(k_scale_cols, grid2(B.m * k, r.n), dim3(256), 0, s, B.U, B.m, (int)k, B.ldu,
+                       (const double *)f.keep, r.dU, r.dws);
             }
-            LAUNCH(k_gauge, dim3(k), dim3(256), 0, s, B.U, B.m, B.ldu, B.Vt, B.n, B.ldv);
+            LAUNCH(k_gauge, dim3(k, r.n), dim3(256), 0, s, B.U, B.m, B.ldu, B.Vt, B.n, B.ldv, r.dU, r.dV);
         } else {
-            R8(B.U, ext(B.m, k, B.ldu), "U");
-            R8(b->keep, k, "keep");
-            LAUNCH(k_scale_cols, dim3(grid_for(B.m * k, 256)), dim3(256), 0, s, B.U, B.m, (int)k, B.ldu,
-                               (const double *)b->keep);
-            LAUNCH(k_gauge, dim3(k), dim3(256), 0, s, B.U, B.m, B.ldu, (double *)nullptr, (int64_t)0, (int64_t)0);
+            R8N(B.U, ext(B.m, k, B.ldu), r.dU, r.n, "U");
+            R8N(f.keep, k, r.dws, r.n, "keep");
+            LAUNCH(k_scale_cols, grid2(B.m * k, r.n), dim3(256), 0, s, B.U, B.m, (int)k, B.ldu, (const double *)f.keep,
+                   r.dU, r.dws);
+            LAUNCH(k_gauge, dim3(k, r.n), dim3(256), 0, s, B.U, B.m, B.ldu, (double *)nullptr, (int64_t)0, (int64_t)0, r.dU,
+                   (int64_t)0);
             if (B.Vt) {
-                R8(B.Vt, ext(k, B.m, B.ldv), "Vt");
-                LAUNCH(k_transpose2d, dim3(grid_for(B.m * k, 256)), dim3(256), 0, s, (const double *)B.U, B.ldu,
-                                   B.Vt, B.ldv, B.m, k);
+                R8N(B.Vt, ext(k, B.m, B.ldv), r.dV, r.n, "Vt");
+                LAUNCH(k_transpose2d, grid2(B.m * k, r.n), dim3(256), 0, s, (const double *)B.U, B.ldu, B.Vt, B.ldv, B.m, k,
+                       r.dU, r.dV);
             }
         }
     }

@@ -158,6 +158,13 @@ typedef struct gtn_gemm_desc {
 } gtn_gemm_desc;
 int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream);
 
+/* Device self-test of the matrix-core operand/result lane layouts the GEMM kernels assume (no
+ * counterpart in the reference: a diagnostic for users of this library).  One wavefront runs each
+ * MFMA form (FP64 16x16x4, FP32 16x16x4, BF16 16x16x32) on inputs whose product is
+ * D[i][j] = 16 i + j + 1 and stores every lane's result registers: out_dev[256 f + 4 lane + v]
+ * (768 doubles).  Under the assumed layout, out_dev[256 f + 4 lane + v] = D[4 (lane / 16) + v][lane % 16]. */
+int gtn_mfma_selftest(double *out_dev, void *stream);
+
 /* ------------------------------------------------------------------------------------------------
  * Reductions and elementwise helpers (FP64 accumulation).
  * gtn_sumsq: out_dev[0] = sum |x|^2 over n elements (Frobenius norm^2; dense.norm __init__.py:888-891,

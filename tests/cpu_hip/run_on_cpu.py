@@ -1,8 +1,8 @@
-"""Run test functions of tests/test_gpu_core.py on the CPU against the AddressSanitizer build of the
-library's HIP sources -- TEST INFRASTRUCTURE ONLY.
+"""Run test functions of a GPU test module (default tests/test_gpu_core.py) on the CPU against the
+AddressSanitizer build of the library's HIP sources -- TEST INFRASTRUCTURE ONLY.
 
   LD_PRELOAD=$(clang++ -print-file-name=libclang_rt.asan-x86_64.so) ASAN_OPTIONS=detect_leaks=0 \\
-      python tests/cpu_hip/run_on_cpu.py [name-substring ...]
+      python tests/cpu_hip/run_on_cpu.py [--module=test_gpu_block] [--exclude=substr] [name-substring ...]
 """
 import inspect
 import os
@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "tests")]
 
 from kernel_emulator import emulate  # noqa: E402
-import test_gpu_core as T  # noqa: E402
+import importlib  # noqa: E402
 
 
 def combos(f):
@@ -33,6 +33,9 @@ def combos(f):
 
 
 def main(args):
+    mods = [a[len("--module="):] for a in args if a.startswith("--module=")]
+    T = importlib.import_module(mods[0] if mods else "test_gpu_core")
+    args = [a for a in args if not a.startswith("--module=")]
     filters = [a for a in args if not a.startswith("--exclude=")]
     excludes = [a[len("--exclude="):] for a in args if a.startswith("--exclude=")]
     lib = os.path.join(HERE, "_build", "libgtn_cpu.so")

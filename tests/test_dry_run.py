@@ -77,3 +77,47 @@ def test_dry_run_catches_a_short_buffer(lib):
     wsa = R.add(ws)
     R.install(lib)
     assert lib.gtn_svd_blocks(arr, 1, C.byref(opts), C.c_void_p(wsa), ws, None) != 0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("m,n,k,count", [(2048, 2048, 32, 4), (128, 128, 8, 70), (512, 512, 16, 65),
+                                         (256, 64, 32, 3), (64, 256, 32, 2), (300, 200, 20, 5)])
+def test_svd_blocks_uniform_runs_geometry(lib, mode, m, n, k, count):
+    """Equal blocks at constant strides form one run; every batched launch stays inside the buffers."""
+    if mode == 1 and m != n:
+        pytest.skip("eig needs square blocks")
+    R = Regions()
+    arr = (L.SvdBlock * count)()
+    lda, ldu, ldv = n, k, n
+    sM, sU, sV = m * n, m * k, k * n
+    M = R.add(8 * sM * count)
+    U = R.add(8 * sU * count)
+    S = R.add(8 * k * count)
+    V = R.add(8 * sV * count)
+    for i in range(count):
+        arr[i].M, arr[i].U, arr[i].S, arr[i].Vt = M + 8 * sM * i, U + 8 * sU * i, S + 8 * k * i, V + 8 * sV * i
+        arr[i].m, arr[i].n, arr[i].lda, arr[i].ldu, arr[i].ldv, arr[i].k = m, n, lda, ldu, ldv, k
+    opts = L.SvdOpts(96, 2, 0, mode, 7)
+    ws = lib.gtn_svd_workspace_bytes(arr, count, C.byref(opts))
+    wsa = R.add(ws)
+    R.install(lib)
+    rc = lib.gtn_svd_blocks(arr, count, C.byref(opts), C.c_void_p(wsa), ws, None)
+    assert rc == 0, lib.gtn_last_error().decode()
+
+
+def test_dry_run_catches_a_short_buffer_at_the_end_of_a_run(lib):
+    R = Regions()
+    count, m, n, k = 6, 256, 256, 32
+    arr = (L.SvdBlock * count)()
+    M = R.add(8 * m * n * count)
+    U = R.add(8 * m * k * count - 8)  # the last block's U is one element short
+    S = R.add(8 * k * count)
+    V = R.add(8 * k * n * count)
+    for i in range(count):
+        arr[i].M, arr[i].U, arr[i].S, arr[i].Vt = M + 8 * m * n * i, U + 8 * m * k * i, S + 8 * k * i, V + 8 * k * n * i
+        arr[i].m, arr[i].n, arr[i].lda, arr[i].ldu, arr[i].ldv, arr[i].k = m, n, n, k, n, k
+    opts = L.SvdOpts(96, 2, 0, 0, 7)
+    ws = lib.gtn_svd_workspace_bytes(arr, count, C.byref(opts))
+    wsa = R.add(ws)
+    R.install(lib)
+    assert lib.gtn_svd_blocks(arr, count, C.byref(opts), C.c_void_p(wsa), ws, None) != 0
