@@ -102,7 +102,8 @@ __device__ __forceinline__ void panel_coord(int s, bool kc, int &r, int &k) {
 // ------------------------------------------------------------------------------------------------
 // FP64 / FP32 kernels: 16x16x4 MFMA, LDS panels stored k-major [BK][ROWS + pad].
 // MFMA 16x16x4 operand layout (per lane l of the wave): A value = A[i = l%16][k = l/16],
-// B value = B[k = l/16][j = l%16]; result lane l holds C[4*(l/16) + v][l%16], v = 0..3.
+// B value = B[k = l/16][j = l%16].  Result registers v = 0..3 of lane l hold C[row][l%16] with
+// row = 4*(l/16) + v for the FP32 form and row = l/16 + 4*v for the FP64 form.
 // ------------------------------------------------------------------------------------------------
 template <typename TIN, typename TC, int BM, int BN, int BK>
 __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
@@ -174,7 +175,8 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
         for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int r = m0 + wm + 16 * i + 4 * (lane / 16) + v;
+                const int row = sizeof(TC) == 8 ? (lane / 16) + 4 * v : 4 * (lane / 16) + v;
+                int r = m0 + wm + 16 * i + row;
                 int c = n0 + wn + 16 * j + lane % 16;
                 if (r < g.M && c < g.N) {
                     int64_t o = r * g.scm + c * g.scn;

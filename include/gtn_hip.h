@@ -162,7 +162,8 @@ int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int p
  * counterpart in the reference: a diagnostic for users of this library).  One wavefront runs each
  * MFMA form (FP64 16x16x4, FP32 16x16x4, BF16 16x16x32) on inputs whose product is
  * D[i][j] = 16 i + j + 1 and stores every lane's result registers: out_dev[256 f + 4 lane + v]
- * (768 doubles).  Under the assumed layout, out_dev[256 f + 4 lane + v] = D[4 (lane / 16) + v][lane % 16]. */
+ * (768 doubles).  The layout the kernels use: out_dev[256 f + 4 lane + v] = D[row][lane % 16] with
+ * row = lane / 16 + 4 v for FP64 and row = 4 (lane / 16) + v for FP32 and BF16. */
 int gtn_mfma_selftest(double *out_dev, void *stream);
 
 /* ------------------------------------------------------------------------------------------------

@@ -90,10 +90,12 @@ typedef double emu_d4 __attribute__((ext_vector_type(4)));
 typedef float emu_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 emu_bf8 __attribute__((ext_vector_type(8)));
 
-template <typename TA, typename TC, int KPL>
+template <typename TA, typename TC, int KPL, bool ROW_INTERLEAVED = false>
 inline TC emu_mfma16(const TA *a, const TA *b, TC c) {
     // lane l supplies A[l%16][KPL*(l/16) + t] and B[KPL*(l/16) + t][l%16], t < KPL;
-    // it receives C[4*(l/16) + v][l%16], v < 4
+    // it holds C[4*(l/16) + v][l%16], v < 4 (FP32 / BF16 forms: 4 consecutive rows per lane), or
+    // C[l/16 + 4*v][l%16] for the FP64 16x16x4 form (ROW_INTERLEAVED: one row per register group;
+    // composable_kernel's mfma_type<mfma_f64_16x16x4f64> has group_size 1, num_groups_per_blk 4)
     struct Slot {
         double a[8], b[8], c[4];
     };
@@ -108,7 +110,7 @@ inline TC emu_mfma16(const TA *a, const TA *b, TC c) {
     emu::sync_wave();
     TC out;
     for (int v = 0; v < 4; ++v) {
-        int row = 4 * (l / 16) + v, col = l % 16;
+        int row = ROW_INTERLEAVED ? (l / 16) + 4 * v : 4 * (l / 16) + v, col = l % 16;
         double acc = buf[l].c[v];
         for (int kk = 0; kk < 4 * KPL; ++kk) {
             int qa = kk / KPL, ta = kk % KPL;
@@ -122,7 +124,7 @@ inline TC emu_mfma16(const TA *a, const TA *b, TC c) {
     return out;
 }
 inline emu_d4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, emu_d4 c, int, int, int) {
-    return emu_mfma16<double, emu_d4, 1>(&a, &b, c);
+    return emu_mfma16<double, emu_d4, 1, true>(&a, &b, c);
 }
 inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, emu_f4 c, int, int, int) {
     return emu_mfma16<float, emu_f4, 1>(&a, &b, c);
