@@ -41,29 +41,36 @@ def site_tensor(d, seed):
     return T / np.abs(T).max()
 
 
-def run_path(gtn, prec, svd_prec, steps, warmup, T0):
+def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
+    """warmup + steps TRG steps; step i runs its contractions in FP64 when i < fp64_steps, else in
+    `prec` (range finder in `svd_prec`).  Only the last `steps` are timed."""
     from grassmanntn_amd._device import Event, synchronize
     from grassmanntn_amd.core import GemmProfile
     T = gtn.dense(T0, encoder="parity-preserving", statistics=(1, 1, -1, -1))
     logn = 0.0
-    lnz = []
-    with gtn.precision(prec, svd_prec):
-        for i in range(warmup):
-            T, nrm = gtn.trg(T, CHI)
+
+    def step(T, i):
+        p = ("f64", "f64") if i < fp64_steps else (prec, svd_prec)
+        with gtn.precision(*p):
+            return gtn.trg(T, CHI)
+
+    for i in range(warmup):
+        T, nrm = step(T, i)
+        logn = 2 * logn + math.log(nrm)
+    synchronize()
+    e0, e1 = Event(), Event()
+    with GemmProfile() as prof:
+        t0 = time.perf_counter()
+        e0.record()
+        for i in range(warmup, warmup + steps):
+            T, nrm = step(T, i)
             logn = 2 * logn + math.log(nrm)
+        e1.record()
         synchronize()
-        e0, e1 = Event(), Event()
-        with GemmProfile() as prof:
-            t0 = time.perf_counter()
-            e0.record()
-            for i in range(warmup, warmup + steps):
-                T, nrm = gtn.trg(T, CHI)
-                logn = 2 * logn + math.log(nrm)
-            e1.record()
-            synchronize()
-            wall = time.perf_counter() - t0
-        gpu_ms = e0.elapsed_ms(e1)
-    F = (gtn.logZ(T) + logn) / 2 ** (warmup + steps)
+        wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_ms(e1)
+    with gtn.precision("f64", "f64"):
+        F = (gtn.logZ(T) + logn) / 2 ** (warmup + steps)
     # T' contraction: two parity-block GEMMs of (chi^2/2) x (D^2/2) x (chi^2/2)
     key = [k for k in prof.ms if k[0] == CHI * CHI // 2 and k[2] == D * D // 2 and k[3] == 2]
     tp = prof.ms[key[0]] if key else []
@@ -107,6 +114,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="bf16x3", help="contraction arithmetic of the mixed path")
     ap.add_argument("--svd-precision", default="f32", help="range-finder arithmetic of the mixed path")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="leading TRG steps of the mixed path run in FP64 (their normalisation errors "
+                         "weigh 2^-n in ln Z per site)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -119,7 +129,7 @@ def main():
         copy_calibration(args.copy_calibration_mib)
         return
     T0 = site_tensor(D, SEED)
-    mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0)
+    mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0, args.fp64_steps)
     fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "trg_golden.json")
@@ -145,7 +155,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "%s contractions, %s range finder, f64 SVD/normalisation" % (args.precision, args.svd_precision),
+        "dtype": "%s contractions, %s range finder, f64 SVD/normalisation (first %d steps all f64)" % (
+            args.precision, args.svd_precision, args.fp64_steps),
         "data": "synthetic",
         "config": {"workload": "TRG, leg dim 64, chi 64, %d steps, seed %d" % (n_total, SEED),
                    "leg_dim": D, "chi": CHI, "steps_total": n_total},
