@@ -41,6 +41,9 @@ def site_tensor(d, seed):
     return T / np.abs(T).max()
 
 
+VERBOSE = False
+
+
 def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
     """warmup + steps TRG steps; step i runs its contractions in FP64 when i < fp64_steps, else in
     `prec` (range finder in `svd_prec`).  Only the last `steps` are timed."""
@@ -51,8 +54,13 @@ def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
 
     def step(T, i):
         p = ("f64", "f64") if i < fp64_steps else (prec, svd_prec)
+        if VERBOSE:
+            print("step %d (%s contractions, %s range finder)" % (i, p[0], p[1]), file=sys.stderr, flush=True)
         with gtn.precision(*p):
-            return gtn.trg(T, CHI)
+            out = gtn.trg(T, CHI)
+        if VERBOSE:
+            print("  Tnorm %.17g" % out[1], file=sys.stderr, flush=True)
+        return out
 
     for i in range(warmup):
         T, nrm = step(T, i)
@@ -119,7 +127,10 @@ def main():
                          "weigh 2^-n in ln Z per site)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
     args = ap.parse_args()
+    global VERBOSE
+    VERBOSE = args.verbose
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs the TRG hot path on a single GPU "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
