@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
+    ap.add_argument("--svd-iters", type=int, default=0, help="subspace iterations of the block SVD "
+                    "(0: the library default)")
     args = ap.parse_args()
     global VERBOSE
     VERBOSE = args.verbose
@@ -139,6 +141,8 @@ def main():
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
         return
+    if args.svd_iters:
+        gtn.settings.svd_iters = args.svd_iters
     T0 = site_tensor(D, SEED)
     mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0, args.fp64_steps)
     fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
@@ -170,7 +174,8 @@ def main():
             args.precision, args.svd_precision, args.fp64_steps),
         "data": "synthetic",
         "config": {"workload": "TRG, leg dim 64, chi 64, %d steps, seed %d" % (n_total, SEED),
-                   "leg_dim": D, "chi": CHI, "steps_total": n_total},
+                   "leg_dim": D, "chi": CHI, "steps_total": n_total, "svd_iters": gtn.settings.svd_iters,
+                   "svd_oversample": gtn.settings.svd_oversample},
         "lnz_per_site_mixed": [mixed["lnz"].real, mixed["lnz"].imag],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": (achieved / peak) if achieved else None, "traffic": None,

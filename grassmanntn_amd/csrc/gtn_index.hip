@@ -151,6 +151,58 @@ __global__ void k_permute_c128(PermArgs a, const double2 *__restrict__ src, doub
     }
 }
 
+// Tiled variant for gathers whose output's last axis is far apart in the source: output axis t1
+// (the one whose unit step moves the source the least) and the last output axis t0 form 64 x 64
+// tiles.  A tile is read with t1 fastest (consecutive source addresses), staged in LDS with its
+// signs applied, and written with t0 fastest (consecutive output addresses).
+struct TileArgs {
+    int32_t t0, t1, n_rest, tiles0, tiles1;
+    int32_t rest_ax[AX];
+    int64_t ostr[AX];
+    int64_t rest_total;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_permute_tiled(PermArgs a, TileArgs t, const T *__restrict__ src,
+                                                       T *__restrict__ dst) {
+    __shared__ double tile[64][65];
+    const double al = a.alpha_re;
+    const int64_t per = (int64_t)t.tiles0 * t.tiles1;
+    const int64_t nblk = per * t.rest_total;
+    const int d1 = (int)a.out_dim[t.t1], d0 = (int)a.out_dim[t.t0];
+    const int64_t s1 = t.ostr[t.t1];
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int64_t r = blk / per;
+        const int64_t q = blk - r * per;
+        const int i1b = (int)(q / t.tiles0) * 64, i0b = (int)(q % t.tiles0) * 64;
+        int64_t base = 0, rr = r;
+        for (int k = t.n_rest - 1; k >= 0; --k) {
+            const int ax = t.rest_ax[k];
+            const int64_t d = a.out_dim[ax];
+            base += (rr % d) * t.ostr[ax];
+            rr /= d;
+        }
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int l1 = e & 63, l0 = e >> 6;
+            const int i1 = i1b + l1, i0 = i0b + l0;
+            double v = 0.0;
+            if (i1 < d1 && i0 < d0) {
+                uint32_t code;
+                const int64_t off = perm_map(a, (uint64_t)(base + i1 * s1 + i0), code);
+                v = (code == 2u) ? 0.0 : (double)src[off] * (code ? -al : al);
+            }
+            tile[l1][l0] = v;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int l0 = e & 63, l1 = e >> 6;
+            const int i1 = i1b + l1, i0 = i0b + l0;
+            if (i1 < d1 && i0 < d0) dst[base + i1 * s1 + i0] = (T)tile[l1][l0];
+        }
+        __syncthreads();
+    }
+}
+
 static int ilog2_exact(uint64_t d) {
     if (d == 0 || (d & (d - 1))) return -1;
     int s = 0;
@@ -459,6 +511,58 @@ int gtn_index_tables(int64_t n, int32_t *popc, int8_t *sigma, int32_t *enc, void
     return GTN_OK;
 }
 
+// Source stride of a unit step of output axis `ax`: through its virtual axis, the least significant
+// component of that axis and the input axis it feeds (0 when it cannot be told: a trace, two virtual
+// axes, a split input).
+static int64_t unit_source_stride(const gtn_perm_desc *d, int ax) {
+    if (d->out_virt[ax][1] >= 0 || d->out_virt[ax][0] < 0) return 0;
+    const int v = d->out_virt[ax][0];
+    int j = 0;
+    for (int u = 0; u <= v; ++u) j += d->virt_ncomp[u];
+    j -= 1;  // the last (least significant) component of v
+    if (j < 0 || d->comp_in[j][1] >= 0) return 0;
+    const int b = d->comp_in[j][0];
+    if (d->in_split[b]) return 0;
+    return d->in_stride[b] * d->comp_mult[j][0] * d->out_vmul[ax][0];
+}
+
+// The tiled kernel pays off when the output's last axis is strided in the source and another output
+// axis of at least 16 entries steps the source by one or two elements.
+static bool plan_tiles(const gtn_perm_desc *d, TileArgs &t) {
+    const int n = d->n_out;
+    if (n < 2) return false;
+    const int t0 = n - 1;
+    if (d->out_dim[t0] < 16) return false;
+    const int64_t s0 = unit_source_stride(d, t0);
+    if (s0 >= 1 && s0 <= 2) return false;  // already coalesced
+    int t1 = -1;
+    for (int ax = 0; ax < t0; ++ax) {
+        const int64_t sa = unit_source_stride(d, ax);
+        if (d->out_dim[ax] >= 16 && sa >= 1 && sa <= 2) {
+            t1 = ax;
+            break;
+        }
+    }
+    if (t1 < 0) return false;
+    memset(&t, 0, sizeof(t));
+    t.t0 = t0;
+    t.t1 = t1;
+    int64_t acc = 1;
+    for (int ax = n - 1; ax >= 0; --ax) {
+        t.ostr[ax] = acc;
+        acc *= d->out_dim[ax];
+    }
+    t.rest_total = 1;
+    for (int ax = 0; ax < n; ++ax)
+        if (ax != t0 && ax != t1) {
+            t.rest_ax[t.n_rest++] = ax;
+            t.rest_total *= d->out_dim[ax];
+        }
+    t.tiles0 = (int32_t)((d->out_dim[t0] + 63) / 64);
+    t.tiles1 = (int32_t)((d->out_dim[t1] + 63) / 64);
+    return true;
+}
+
 int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream) {
     GTN_REQUIRE(d && src && dst, "gtn_permute: null argument");
     PermArgs a;
@@ -471,8 +575,19 @@ int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream
         GTN_RANGE(dst, a.total * es, "permute dst");
         return GTN_OK;
     }
-    dim3 g(grid_for(a.total, 256, 1 << 20)), b(256);
     hipStream_t s = as_stream(stream);
+    TileArgs t;
+    if (d->dtype != GTN_C128 && plan_tiles(d, t)) {
+        const int64_t nblk = (int64_t)t.tiles0 * t.tiles1 * t.rest_total;
+        dim3 gt((unsigned)(nblk < (1 << 20) ? nblk : (1 << 20))), bt(256);
+        if (d->dtype == GTN_F64)
+            hipLaunchKernelGGL(k_permute_tiled<double>, gt, bt, 0, s, a, t, (const double *)src, (double *)dst);
+        else
+            hipLaunchKernelGGL(k_permute_tiled<float>, gt, bt, 0, s, a, t, (const float *)src, (float *)dst);
+        GTN_CHECK_LAUNCH();
+        return GTN_OK;
+    }
+    dim3 g(grid_for(a.total, 256, 1 << 20)), b(256);
     switch (d->dtype) {
         case GTN_F64: hipLaunchKernelGGL(k_permute<double>, g, b, 0, s, a, (const double *)src, (double *)dst); break;
         case GTN_F32: hipLaunchKernelGGL(k_permute<float>, g, b, 0, s, a, (const float *)src, (float *)dst); break;

@@ -149,11 +149,11 @@ __device__ double wg_sum(double v, double *sh) {
     return t;
 }
 
-// Cholesky W = R^T R (W p x p SPD, shifted by shift_rel * trace / p), then out = R^{-1} (full
-// p x p, zero below the diagonal).  R and R^{-1} are kept in LDS in packed upper-triangular form
-// (2 x p(p+1)/2 doubles, 132 KiB at p = 128).  Pivots that fall to <= 0 are replaced by a tiny
-// positive value so that rank-deficient inputs still give finite columns (they are removed later
-// by the rank rule).
+// Cholesky W = R^T R (W p x p symmetric positive semi-definite, shifted by shift_rel * trace / p),
+// then out = R^{-1} (full p x p, zero below the diagonal).  R is kept in LDS in packed upper form
+// (p(p+1)/2 doubles).  A pivot at or below 1e-14 of the largest diagonal entry marks its column as
+// dependent: its row of R and its row and column of R^{-1} are zero, so that a rank-deficient
+// basis gives zero columns instead of overflowing ones.
 __device__ __forceinline__ int up_idx(int i, int j, int p) { return i * p - (i * (i - 1)) / 2 + (j - i); }
 
 __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
@@ -163,30 +163,36 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     const int p = P.p;
     const double *W = P.in;
     double *Rinv = P.out;
-    const int np = p * (p + 1) / 2;
-    double *A = sm;       // packed upper R
-    double *X = sm + np;  // packed upper R^{-1}
+    double *A = sm;  // packed upper R
     const int tid = threadIdx.x, nt = blockDim.x;
-    double tr = 0;
+    double tr = 0, dm = 0;
     for (int e = tid; e < p * p; e += nt) {
         int i = e / p, j = e - i * p;
         if (j >= i) A[up_idx(i, j, p)] = W[e];
-        if (i == j) tr += W[e];
+        if (i == j) {
+            tr += W[e];
+            dm = fmax(dm, W[e]);
+        }
     }
     tr = wg_sum(tr, red);
+    // maximum of the diagonal: every thread's dm through the same reduction buffer
+    for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = dm;
+    __syncthreads();
+    dm = 0;
+    for (int i = 0; i < (int)(nt >> 6); ++i) dm = fmax(dm, red[i]);
+    __syncthreads();
     const double shift = B.shift_rel * tr / p;
-    const double floor_v = 1e-300 + 1e-30 * (tr / p);
+    const double dead_tol = 1e-14 * (dm + shift);
     for (int i = tid; i < p; i += nt) A[up_idx(i, i, p)] += shift;
     __syncthreads();
     for (int k = 0; k < p; ++k) {
-        if (tid == 0) {
-            double d = A[up_idx(k, k, p)];
-            A[up_idx(k, k, p)] = sqrt(d > floor_v ? d : floor_v);
-        }
+        const double d = A[up_idx(k, k, p)];
+        const bool live = d > dead_tol;  // false for NaN as well
+        const double inv = live ? 1.0 / sqrt(d) : 0.0;
+        for (int j = k + 1 + tid; j < p; j += nt) A[up_idx(k, j, p)] *= inv;
         __syncthreads();
-        const double rkk = A[up_idx(k, k, p)];
-        for (int j = k + 1 + tid; j < p; j += nt) A[up_idx(k, j, p)] /= rkk;
-        __syncthreads();
+        if (tid == 0) A[up_idx(k, k, p)] = live ? sqrt(d) : 0.0;
         const int rem = p - k - 1;
         for (int e = tid; e < rem * rem; e += nt) {
             int i = k + 1 + e / rem, j = k + 1 + e % rem;
@@ -194,32 +200,59 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
         }
         __syncthreads();
     }
-    // X = R^{-1} by back substitution, one column per thread
-    for (int c = tid; c < p; c += nt) {
-        for (int i = c; i >= 0; --i) {
-            double s = (i == c) ? 1.0 : 0.0;
-            for (int j = i + 1; j <= c; ++j) s -= A[up_idx(i, j, p)] * X[up_idx(j, c, p)];
-            X[up_idx(i, c, p)] = s / A[up_idx(i, i, p)];
+    // X = R^{-1}, column c by back substitution over rows i = c .. 0 by a group of 8 lanes: lane q
+    // keeps X[j][c] for j = 8 jj + q in registers and the dot products are reduced across the group
+    constexpr int G = 8, JJ = kPMax / G;
+    const int q = tid % G, c = tid / G;
+    double xs[JJ];
+#pragma unroll
+    for (int jj = 0; jj < JJ; ++jj) xs[jj] = 0.0;
+    // every lane runs the same p steps (uniform shuffles); steps with i > c or c >= p do nothing
+    const bool col = c < p;
+    for (int i = p - 1; i >= 0; --i) {
+        double part = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < JJ; ++jj) {
+            const int j = jj * G + q;
+            if (col && j > i && j <= c) part += A[up_idx(i, j, p)] * xs[jj];
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
+        if (col && i <= c && (i % G) == q) {
+            const double rii = A[up_idx(i, i, p)];
+            const double xic = (rii > 0.0) ? (((i == c) ? 1.0 : 0.0) - part) / rii : 0.0;
+#pragma unroll
+            for (int jj = 0; jj < JJ; ++jj)
+                if (jj == i / G) xs[jj] = xic;
         }
     }
-    __syncthreads();
-    for (int e = tid; e < p * p; e += nt) {
-        int i = e / p, j = e - i * p;
-        Rinv[e] = (j >= i) ? X[up_idx(i, j, p)] : 0.0;
+    // write column c of R^{-1} (rows j <= c from the registers, zero below the diagonal)
+    if (c < p) {
+#pragma unroll
+        for (int jj = 0; jj < JJ; ++jj) {
+            const int j = jj * G + q;
+            if (j < p) Rinv[(int64_t)j * p + c] = (j <= c) ? xs[jj] : 0.0;
+        }
     }
 }
 
 // Cyclic Jacobi on a symmetric p x p matrix (p even) in LDS.  Round-robin pairing gives p/2
-// disjoint rotations per round and every pair once per sweep.  Eigenvectors accumulate in
-// global memory (aux, p x p, column e = eigenvector e); on exit lam and the columns of `out` are
+// disjoint rotations per round and every pair once per sweep.  Each round computes its rotations,
+// then applies both sides at once: thread e owns the 2 x 2 block of rows {i_a, j_a} x columns
+// {i_b, j_b} for the pair of rotations (a, b), so every entry is written by one thread.  The
+// eigenvectors accumulate transposed in global memory (aux: row e = eigenvector e, so that the
+// update of two eigenvectors is a contiguous read and write).  A rotation is skipped when
+// |a_ij| <= 1e-15 sqrt(|a_ii a_jj|) or |a_ij| <= 1e-17 ||A||; the sweeps stop when a whole sweep
+// skips every rotation.  On exit lam and the columns of `out` (p x p, column e = eigenvector e) are
 // sorted by decreasing lambda (or |lambda| when by_abs).
 __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
     __shared__ double red[16];
+    __shared__ int active;
     const SmallPtrs P = small_ptrs(B);
     const int p = P.p;
     const double *Cin = P.in;
-    double *V = P.aux;
+    double *Vt = P.aux;
     double *Vout = P.out;
     double *lam = P.lam;
     double *A = sm;
@@ -233,16 +266,12 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
         double v = Cin[i];
         A[i] = v;
         nrm += v * v;
-        V[i] = (i / p == i % p) ? 1.0 : 0.0;
+        Vt[i] = (i / p == i % p) ? 1.0 : 0.0;
     }
+    if (tid == 0) active = 0;
     nrm = wg_sum(nrm, red);
-    const double tol = 1e-32 * nrm;
+    const double abs_tol = 1e-17 * sqrt(nrm);
     for (int sweep = 0; sweep < B.max_sweeps; ++sweep) {
-        double off = 0;
-        for (int i = tid; i < p * p; i += nt)
-            if (i / p != i % p) off += A[i] * A[i];
-        off = wg_sum(off, red);
-        if (off <= tol) break;
         for (int round = 0; round < p - 1; ++round) {
             for (int t = tid; t < h; t += nt) {
                 int a = (t == 0) ? 0 : ((t - 1 + round) % (p - 1)) + 1;
@@ -250,11 +279,13 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
                 int i = a < b ? a : b, j = a < b ? b : a;
                 double aii = A[i * p + i], ajj = A[j * p + j], aij = A[i * p + j];
                 double c = 1.0, s = 0.0;
-                if (fabs(aij) > 1e-300) {
+                const double fa = fabs(aij);
+                if (fa > abs_tol && fa > 1e-15 * sqrt(fabs(aii * ajj))) {
                     double tau = (ajj - aii) / (2.0 * aij);
                     double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                     c = 1.0 / sqrt(1.0 + tt * tt);
                     s = tt * c;
+                    active = 1;
                 }
                 cs[t] = c;
                 cs[h + t] = s;
@@ -262,33 +293,33 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
                 pr[h + t] = j;
             }
             __syncthreads();
-            for (int e = tid; e < h * p; e += nt) {  // rows: A <- J^T A
-                int t = e / p, col = e - t * p;
-                int i = pr[t], j = pr[h + t];
-                double c = cs[t], s = cs[h + t];
-                double ai = A[i * p + col], aj = A[j * p + col];
-                A[i * p + col] = c * ai - s * aj;
-                A[j * p + col] = s * ai + c * aj;
+            for (int e = tid; e < h * h; e += nt) {  // A <- J^T A J, one 2 x 2 block per thread
+                const int ta = e / h, tb = e - ta * h;
+                const int i = pr[ta], j = pr[h + ta], k = pr[tb], l = pr[h + tb];
+                const double ca = cs[ta], sa = cs[h + ta], cb = cs[tb], sb = cs[h + tb];
+                const double xik = A[i * p + k], xil = A[i * p + l], xjk = A[j * p + k], xjl = A[j * p + l];
+                const double rik = ca * xik - sa * xjk, rjk = sa * xik + ca * xjk;
+                const double ril = ca * xil - sa * xjl, rjl = sa * xil + ca * xjl;
+                A[i * p + k] = cb * rik - sb * ril;
+                A[i * p + l] = sb * rik + cb * ril;
+                A[j * p + k] = cb * rjk - sb * rjl;
+                A[j * p + l] = sb * rjk + cb * rjl;
             }
-            __syncthreads();
-            for (int e = tid; e < h * p; e += nt) {  // columns: A <- A J, V <- V J
-                int t = e / p, row = e - t * p;
-                int i = pr[t], j = pr[h + t];
-                double c = cs[t], s = cs[h + t];
-                double ai = A[row * p + i], aj = A[row * p + j];
-                A[row * p + i] = c * ai - s * aj;
-                A[row * p + j] = s * ai + c * aj;
-                double vi = V[row * p + i], vj = V[row * p + j];
-                V[row * p + i] = c * vi - s * vj;
-                V[row * p + j] = s * vi + c * vj;
-            }
-            __syncthreads();
-            for (int t = tid; t < h; t += nt) {
-                A[pr[t] * p + pr[h + t]] = 0.0;
-                A[pr[h + t] * p + pr[t]] = 0.0;
+            for (int e = tid; e < h * p; e += nt) {  // V <- V J (rows i, j of V^T)
+                const int t = e / p, r = e - t * p;
+                const int i = pr[t], j = pr[h + t];
+                const double c = cs[t], s = cs[h + t];
+                const double vi = Vt[(int64_t)i * p + r], vj = Vt[(int64_t)j * p + r];
+                Vt[(int64_t)i * p + r] = c * vi - s * vj;
+                Vt[(int64_t)j * p + r] = s * vi + c * vj;
             }
             __syncthreads();
         }
+        const int go = active;
+        __syncthreads();
+        if (!go) break;
+        if (tid == 0) active = 0;
+        __syncthreads();
     }
     // sort keys: lambda (or |lambda|), with NaN ranked below everything, ties broken by index, so
     // that the ranks are a permutation of 0..p-1 whatever the values
@@ -309,7 +340,7 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     for (int i = tid; i < p; i += nt) lam[i] = A[order[i] * p + order[i]];
     for (int i = tid; i < p * p; i += nt) {
         int r = i / p, c = i - r * p;
-        Vout[i] = V[r * p + order[c]];
+        Vout[i] = Vt[(int64_t)order[c] * p + r];
     }
 }
 
@@ -687,7 +718,7 @@ static int orthonormalise(std::vector<Run> &runs, int passes, hipStream_t s) {
                 sb.in[i] = b.W;
                 sb.out[i] = b.R;
             },
-            k_chol_inv, sizeof(double), sizeof(double), proto, s);
+            k_chol_inv, sizeof(double) / 2, sizeof(double), proto, s);
         if (rc) return rc;
         std::vector<gtn_gemm_desc> g;
         for (Run &r : runs) {

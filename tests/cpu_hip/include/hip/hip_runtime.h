@@ -83,6 +83,21 @@ inline T __shfl_down(T v, unsigned delta, int width = 64) {
     return r;
 }
 
+// butterfly exchange within segments of `width` lanes (width a power of two, mask < width)
+template <typename T>
+inline T __shfl_xor(T v, int mask, int width = 64) {
+    static_assert(sizeof(T) <= 8, "shuffle of a value wider than 8 bytes");
+    T *buf = (T *)emu::wave_buf(0);
+    int l = emu::lane();
+    emu::sync_wave();
+    buf[l] = v;
+    emu::sync_wave();
+    int src = l ^ mask;
+    T r = ((src / width) == (l / width)) ? buf[src] : v;
+    emu::sync_wave();
+    return r;
+}
+
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 // ---- MFMA emulation (wave64; operand/result layouts of the 16x16 instructions) -------------------
