@@ -69,7 +69,7 @@ class SvdBlock(C.Structure):
 class SvdOpts(C.Structure):
     _fields_ = [
         ("oversample", C.c_int32), ("iters", C.c_int32), ("precision", C.c_int32), ("mode", C.c_int32),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("orth_every", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

@@ -34,7 +34,8 @@ class _Settings:
     precision = "f64"       # GEMM arithmetic of einsum: f64 | f32 | bf16x3 | bf16x6
     svd_precision = "f64"   # range-finder GEMMs of the block SVD
     svd_oversample = 96
-    svd_iters = 10
+    svd_iters = 18          # subspace iterations: error ~ (sigma_{p+1} / sigma_k)^(2 iters)
+    svd_orth_every = 3      # CholeskyQR of the basis every this many iterations
     seed = 12345
 
 
@@ -1034,6 +1035,7 @@ def _svd_blocks_raw(blocks, mode):
     opts.precision = L.PREC[settings.svd_precision]
     opts.mode = mode
     opts.seed = settings.seed
+    opts.orth_every = settings.svd_orth_every
     ws_bytes = lib.gtn_svd_workspace_bytes(arr, nblk, C.byref(opts))
     ws = DeviceArray.empty(max(ws_bytes // 8, 1), np.float64)
     L.check(lib.gtn_svd_blocks(arr, nblk, C.byref(opts), L.ptr(ws), ws_bytes, L.stream_ptr()), "gtn_svd_blocks")

@@ -32,6 +32,7 @@ static inline int64_t ext(int64_t rows, int64_t cols, int64_t ld) { return rows 
 
 constexpr int kPMax = 128;   // largest p x p problem held in LDS (128 KiB of FP64)
 constexpr int kMaxBatch = 8; // blocks per launch of the LDS kernels
+constexpr int kMaxSweeps = 30;  // Jacobi sweep limit (the rotation log is sized for it)
 
 // ---------------------------------------------------------------------------------------------
 // counter-based Gaussian fill: a 64-bit mix of (seed, index) -> two uniforms -> Box-Muller
@@ -236,15 +237,23 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     }
 }
 
+// round-robin pairing of round `round` (p even): pair t is (a, b) with a != b, every index once
+__device__ __forceinline__ void rr_pair(int t, int round, int p, int &i, int &j) {
+    const int a = (t == 0) ? 0 : ((t - 1 + round) % (p - 1)) + 1;
+    const int b = ((p - 2 - t + round) % (p - 1)) + 1;
+    i = a < b ? a : b;
+    j = a < b ? b : a;
+}
+
 // Cyclic Jacobi on a symmetric p x p matrix (p even) in LDS.  Round-robin pairing gives p/2
 // disjoint rotations per round and every pair once per sweep.  Each round computes its rotations,
 // then applies both sides at once: thread e owns the 2 x 2 block of rows {i_a, j_a} x columns
 // {i_b, j_b} for the pair of rotations (a, b), so every entry is written by one thread.  The
-// eigenvectors accumulate transposed in global memory (aux: row e = eigenvector e, so that the
-// update of two eigenvectors is a contiguous read and write).  A rotation is skipped when
+// rotations (c, s) are appended to a log (in: p/2 pairs per round, pairs implied by the round);
+// k_jacobi_vectors multiplies them out into the eigenvectors afterwards.  A rotation is skipped when
 // |a_ij| <= 1e-15 sqrt(|a_ii a_jj|) or |a_ij| <= 1e-17 ||A||; the sweeps stop when a whole sweep
-// skips every rotation.  On exit lam and the columns of `out` (p x p, column e = eigenvector e) are
-// sorted by decreasing lambda (or |lambda| when by_abs).
+// skips every rotation.  On exit lam holds the eigenvalues sorted by decreasing lambda (or
+// |lambda| when by_abs); aux[0] = rounds logged, aux[1 + c] = index of the c-th sorted eigenvalue.
 __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
     __shared__ double red[16];
@@ -252,8 +261,8 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     const SmallPtrs P = small_ptrs(B);
     const int p = P.p;
     const double *Cin = P.in;
-    double *Vt = P.aux;
-    double *Vout = P.out;
+    double *info = P.aux;
+    double *lg = P.out;
     double *lam = P.lam;
     double *A = sm;
     double *cs = sm + p * p;
@@ -266,17 +275,16 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
         double v = Cin[i];
         A[i] = v;
         nrm += v * v;
-        Vt[i] = (i / p == i % p) ? 1.0 : 0.0;
     }
     if (tid == 0) active = 0;
     nrm = wg_sum(nrm, red);
     const double abs_tol = 1e-17 * sqrt(nrm);
+    int rounds = 0;
     for (int sweep = 0; sweep < B.max_sweeps; ++sweep) {
-        for (int round = 0; round < p - 1; ++round) {
+        for (int round = 0; round < p - 1; ++round, ++rounds) {
             for (int t = tid; t < h; t += nt) {
-                int a = (t == 0) ? 0 : ((t - 1 + round) % (p - 1)) + 1;
-                int b = ((p - 2 - t + round) % (p - 1)) + 1;
-                int i = a < b ? a : b, j = a < b ? b : a;
+                int i, j;
+                rr_pair(t, round, p, i, j);
                 double aii = A[i * p + i], ajj = A[j * p + j], aij = A[i * p + j];
                 double c = 1.0, s = 0.0;
                 const double fa = fabs(aij);
@@ -291,6 +299,8 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
                 cs[h + t] = s;
                 pr[t] = i;
                 pr[h + t] = j;
+                lg[((int64_t)rounds * h + t) * 2] = c;
+                lg[((int64_t)rounds * h + t) * 2 + 1] = s;
             }
             __syncthreads();
             for (int e = tid; e < h * h; e += nt) {  // A <- J^T A J, one 2 x 2 block per thread
@@ -304,14 +314,6 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
                 A[i * p + l] = sb * rik + cb * ril;
                 A[j * p + k] = cb * rjk - sb * rjl;
                 A[j * p + l] = sb * rjk + cb * rjl;
-            }
-            for (int e = tid; e < h * p; e += nt) {  // V <- V J (rows i, j of V^T)
-                const int t = e / p, r = e - t * p;
-                const int i = pr[t], j = pr[h + t];
-                const double c = cs[t], s = cs[h + t];
-                const double vi = Vt[(int64_t)i * p + r], vj = Vt[(int64_t)j * p + r];
-                Vt[(int64_t)i * p + r] = c * vi - s * vj;
-                Vt[(int64_t)j * p + r] = s * vi + c * vj;
             }
             __syncthreads();
         }
@@ -337,10 +339,60 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
         order[rank] = e;
     }
     __syncthreads();
-    for (int i = tid; i < p; i += nt) lam[i] = A[order[i] * p + order[i]];
-    for (int i = tid; i < p * p; i += nt) {
-        int r = i / p, c = i - r * p;
-        Vout[i] = Vt[(int64_t)order[c] * p + r];
+    for (int i = tid; i < p; i += nt) {
+        lam[i] = A[order[i] * p + order[i]];
+        info[1 + i] = (double)order[i];
+    }
+    if (tid == 0) info[0] = (double)rounds;
+}
+
+// Eigenvectors from the rotation log of k_jacobi: V = J_1 J_2 ... J_R applied to the identity.  Rows
+// of V evolve independently (v <- v J), so workgroup (x, y) handles rows 8x .. 8x+7 of block y: one
+// thread per (row, pair), one barrier per round.  out (p x p): column c = eigenvector of the c-th
+// sorted eigenvalue.  Launch: grid (ceil(p / 8), blocks), 8 * (p / 2) threads.
+__global__ void __launch_bounds__(512) k_jacobi_vectors(SmallBatch B) {
+    __shared__ double rows[8][kPMax + 1];
+    SmallPtrs P;
+    const int64_t bi = blockIdx.y;
+    if (B.uniform) {
+        P.p = B.p[0];
+        P.in = B.in[0] + bi * B.d_in;
+        P.out = B.out[0] + bi * B.d_out;
+        P.aux = B.aux[0] + bi * B.d_aux;
+    } else {
+        P.p = B.p[bi];
+        P.in = B.in[bi];
+        P.out = B.out[bi];
+        P.aux = B.aux[bi];
+    }
+    const int p = P.p, h = p / 2;
+    const double *lg = P.in;
+    const double *info = P.aux;
+    const int r0 = blockIdx.x * 8;
+    const int q = threadIdx.x / h, t = threadIdx.x - q * h;  // row q of the chunk, pair t
+    const bool live = threadIdx.x < 8 * h && r0 + q < p;
+    for (int e = threadIdx.x; e < 8 * p; e += blockDim.x) {
+        const int rq = e / p, c = e - rq * p;
+        rows[rq][c] = (r0 + rq == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int R = (int)info[0];
+    for (int r = 0; r < R; ++r) {
+        if (live) {
+            int i, j;
+            rr_pair(t, r % (p - 1), p, i, j);
+            const double c = lg[((int64_t)r * h + t) * 2], s = lg[((int64_t)r * h + t) * 2 + 1];
+            if (s != 0.0) {
+                const double vi = rows[q][i], vj = rows[q][j];
+                rows[q][i] = c * vi - s * vj;
+                rows[q][j] = s * vi + c * vj;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < 8 * p; e += blockDim.x) {
+        const int rq = e / p, c = e - rq * p;
+        if (r0 + rq < p) P.out[(int64_t)(r0 + rq) * p + c] = rows[rq][(int)info[1 + c]];
     }
 }
 
@@ -430,7 +482,7 @@ struct Blk {
     int64_t p;
     bool small, left;
     double *ws;
-    double *Y, *Z, *T, *Bm, *W, *G, *R, *V, *Vs, *lam, *inv, *keep, *X, *part;
+    double *Y, *Z, *T, *Bm, *W, *G, *R, *V, *Vs, *lam, *inv, *keep, *X, *part, *Lg;
     int64_t bytes;
 };
 
@@ -468,6 +520,7 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
     int64_t oW = take(p * p), oG = take(p * p), oR = take(p * p), oV = take(p * p), oVs = take(p * p);
     int64_t ol = take(p), oi = take(p), ok = take(p), oX = take(big * kk);
     int64_t opart = take((int64_t)32 * p * p);  // up to 32 split-K partial Gram matrices
+    int64_t olg = take((int64_t)kMaxSweeps * (p - 1) * p);  // Jacobi rotation log: (c, s) per pair
     P.bytes = (off * 8 + 255) / 256 * 256;
     P.ws = ws;
     if (ws) {
@@ -485,6 +538,7 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
         P.keep = ws + ok;
         P.X = ws + oX;
         P.part = ws + opart;
+        P.Lg = ws + olg;
     }
 }
 
@@ -743,17 +797,38 @@ static int jacobi(std::vector<Run> &runs, int by_abs, hipStream_t s) {
     SmallBatch proto;
     memset(&proto, 0, sizeof(proto));
     proto.by_abs = by_abs;
-    proto.max_sweeps = 30;
-    return small_launch(
+    proto.max_sweeps = kMaxSweeps;
+    // eigenvalues, order and the rotation log (out = log, aux = info)
+    int rc = small_launch(
         runs,
         [](Blk &b, SmallBatch &sb, int i) {
             sb.p[i] = (int)b.p;
             sb.in[i] = b.G;
-            sb.out[i] = b.Vs;
+            sb.out[i] = b.Lg;
             sb.aux[i] = b.V;
             sb.lam[i] = b.lam;
         },
         k_jacobi, sizeof(double), sizeof(double) + 3 * sizeof(int), proto, s);
+    if (rc) return rc;
+    // eigenvectors: one launch per run (uniform) or per single block
+    for (Run &r : runs) {
+        Blk &f = *r.f;
+        const int64_t p = f.p;
+        R8N(f.Lg, (int64_t)kMaxSweeps * (p - 1) * p, r.dws, r.n, "rotation log");
+        R8N(f.V, p + 1, r.dws, r.n, "jacobi info");
+        R8N(f.Vs, p * p, r.dws, r.n, "eigenvectors");
+        SmallBatch sb;
+        memset(&sb, 0, sizeof(sb));
+        sb.n = r.n;
+        sb.uniform = 1;
+        sb.p[0] = (int)p;
+        sb.in[0] = f.Lg;
+        sb.out[0] = f.Vs;
+        sb.aux[0] = f.V;
+        sb.d_in = sb.d_out = sb.d_aux = r.dws;
+        LAUNCH(k_jacobi_vectors, dim3((unsigned)((p + 7) / 8), r.n), dim3((unsigned)(8 * (p / 2))), 0, s, sb);
+    }
+    return GTN_OK;
 }
 
 static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream_t s) {
@@ -840,7 +915,12 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
                            f.b.m, f.p, r.dws, r.dws);
                 }
             }
-            rc = orthonormalise(rl, it + 1 == o.iters ? 2 : 1, s);
+            const int every = o.orth_every > 1 ? o.orth_every : 1;
+            if (it + 1 == o.iters) {
+                rc = orthonormalise(rl, 2, s);
+            } else if ((it + 1) % every == 0) {
+                rc = orthonormalise(rl, 1, s);
+            }
             if (rc) return rc;
         }
         // Rayleigh-Ritz in FP64: Bm = Y^T M (p x n); G = Bm Bm^T (svd) or Bm Y (eig)

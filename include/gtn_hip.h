@@ -217,6 +217,8 @@ typedef struct gtn_svd_opts {
     int32_t precision;  /* GTN_PREC_* for the range finder GEMMs */
     int32_t mode;       /* 0: SVD, 1: Hermitian eig */
     uint64_t seed;
+    int32_t orth_every; /* re-orthonormalise the basis every this many iterations (<= 1: every one) */
+    int32_t pad_;
 } gtn_svd_opts;
 int64_t gtn_svd_workspace_bytes(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts);
 int gtn_svd_blocks(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts, void *workspace_dev,
