@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
     ap.add_argument("--svd-iters", type=int, default=0, help="subspace iterations of the block SVD "
                     "(0: the library default)")
+    ap.add_argument("--svd-orth-every", type=int, default=0, help="re-orthonormalise the range-finder "
+                    "basis every this many iterations (0: the library default)")
     args = ap.parse_args()
     global VERBOSE
     VERBOSE = args.verbose
@@ -143,6 +145,8 @@ def main():
         return
     if args.svd_iters:
         gtn.settings.svd_iters = args.svd_iters
+    if args.svd_orth_every:
+        gtn.settings.svd_orth_every = args.svd_orth_every
     T0 = site_tensor(D, SEED)
     mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0, args.fp64_steps)
     fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
@@ -175,7 +179,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "TRG, leg dim 64, chi 64, %d steps, seed %d" % (n_total, SEED),
                    "leg_dim": D, "chi": CHI, "steps_total": n_total, "svd_iters": gtn.settings.svd_iters,
-                   "svd_oversample": gtn.settings.svd_oversample},
+                   "svd_oversample": gtn.settings.svd_oversample,
+                   "svd_orth_every": gtn.settings.svd_orth_every},
         "lnz_per_site_mixed": [mixed["lnz"].real, mixed["lnz"].imag],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": (achieved / peak) if achieved else None, "traffic": None,
