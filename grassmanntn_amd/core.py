@@ -34,8 +34,8 @@ class _Settings:
     precision = "f64"       # GEMM arithmetic of einsum: f64 | f32 | bf16x3 | bf16x6
     svd_precision = "f64"   # range-finder GEMMs of the block SVD
     svd_oversample = 96
-    svd_iters = 18          # subspace iterations: error ~ (sigma_{p+1} / sigma_k)^(2 iters)
-    svd_orth_every = 3      # CholeskyQR of the basis every this many iterations
+    svd_iters = 20          # subspace iterations: error ~ (sigma_{p+1} / sigma_k)^(2 iters)
+    svd_orth_every = 2      # CholeskyQR of the basis every this many iterations
     seed = 12345
 
 

@@ -151,11 +151,16 @@ __device__ double wg_sum(double v, double *sh) {
 }
 
 // Cholesky W = R^T R (W p x p symmetric positive semi-definite, shifted by shift_rel * trace / p),
-// then out = R^{-1} (full p x p, zero below the diagonal).  R is kept in LDS in packed upper form
-// (p(p+1)/2 doubles).  A pivot at or below 1e-14 of the largest diagonal entry marks its column as
-// dependent: its row of R and its row and column of R^{-1} are zero, so that a rank-deficient
-// basis gives zero columns instead of overflowing ones.
+// then out = R^{-1} (full p x p, zero below the diagonal).  Both run blocked in 16-wide panels on the
+// full matrix in LDS (p^2 + 16 p doubles): a wavefront factors each diagonal block, every thread
+// solves one column of the panel rows, the trailing update is a 16-deep rank update; the inverse
+// inverts the diagonal blocks and then fills block columns left to right (two barriers each).  A
+// pivot at or below 1e-14 of the largest diagonal entry marks its column as dependent: its row of R
+// and its row and column of R^{-1} are zero, so that a rank-deficient basis gives zero columns
+// instead of overflowing ones.
 __device__ __forceinline__ int up_idx(int i, int j, int p) { return i * p - (i * (i - 1)) / 2 + (j - i); }
+
+constexpr int kNB = 16;  // panel width of the blocked Cholesky / triangular inverse
 
 __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
@@ -164,19 +169,20 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     const int p = P.p;
     const double *W = P.in;
     double *Rinv = P.out;
-    double *A = sm;  // packed upper R
+    double *A = sm;          // p x p, upper triangle: R, then R^{-1} in place
+    double *T = sm + p * p;  // p x kNB scratch
     const int tid = threadIdx.x, nt = blockDim.x;
     double tr = 0, dm = 0;
     for (int e = tid; e < p * p; e += nt) {
-        int i = e / p, j = e - i * p;
-        if (j >= i) A[up_idx(i, j, p)] = W[e];
+        const int i = e / p, j = e - i * p;
+        const double w = W[e];
+        A[e] = (j >= i) ? w : 0.0;
         if (i == j) {
-            tr += W[e];
-            dm = fmax(dm, W[e]);
+            tr += w;
+            dm = fmax(dm, w);
         }
     }
     tr = wg_sum(tr, red);
-    // maximum of the diagonal: every thread's dm through the same reduction buffer
     for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
     if ((tid & 63) == 0) red[tid >> 6] = dm;
     __syncthreads();
@@ -185,55 +191,96 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     __syncthreads();
     const double shift = B.shift_rel * tr / p;
     const double dead_tol = 1e-14 * (dm + shift);
-    for (int i = tid; i < p; i += nt) A[up_idx(i, i, p)] += shift;
+    for (int i = tid; i < p; i += nt) A[i * p + i] += shift;
     __syncthreads();
-    for (int k = 0; k < p; ++k) {
-        const double d = A[up_idx(k, k, p)];
-        const bool live = d > dead_tol;  // false for NaN as well
-        const double inv = live ? 1.0 / sqrt(d) : 0.0;
-        for (int j = k + 1 + tid; j < p; j += nt) A[up_idx(k, j, p)] *= inv;
+    const int nblk = (p + kNB - 1) / kNB;
+    // ---- right-looking blocked Cholesky W = R^T R
+    for (int J = 0; J < nblk; ++J) {
+        const int j0 = J * kNB, j1 = (j0 + kNB < p) ? j0 + kNB : p;
+        if (tid < 64) {  // the diagonal block, by one wavefront (its LDS traffic stays in order)
+            for (int k = j0; k < j1; ++k) {
+                const double d = A[k * p + k];
+                const bool live = d > dead_tol;  // false for NaN as well
+                const double inv = live ? 1.0 / sqrt(d) : 0.0;
+                const int j = k + 1 + tid;
+                if (j < j1) A[k * p + j] *= inv;
+                __builtin_amdgcn_wave_barrier();
+                if (tid == 0) A[k * p + k] = live ? sqrt(d) : 0.0;
+                const int rem = j1 - k - 1;
+                for (int e = tid; e < rem * rem; e += 64) {
+                    const int i = k + 1 + e / rem, jj = k + 1 + e % rem;
+                    if (jj >= i) A[i * p + jj] -= A[k * p + i] * A[k * p + jj];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
         __syncthreads();
-        if (tid == 0) A[up_idx(k, k, p)] = live ? sqrt(d) : 0.0;
-        const int rem = p - k - 1;
+        // panel rows j0..j1-1 right of the block: R_JJ^T X = A_J,c, one column per thread
+        for (int c = j1 + tid; c < p; c += nt) {
+            for (int r = j0; r < j1; ++r) {
+                double v = A[r * p + c];
+                for (int q = j0; q < r; ++q) v -= A[q * p + r] * A[q * p + c];
+                const double rr = A[r * p + r];
+                A[r * p + c] = rr > 0.0 ? v / rr : 0.0;
+            }
+        }
+        __syncthreads();
+        // trailing update of the upper triangle: A_ij -= sum over the panel rows of R_ri R_rj
+        const int rem = p - j1;
         for (int e = tid; e < rem * rem; e += nt) {
-            int i = k + 1 + e / rem, j = k + 1 + e % rem;
-            if (j >= i) A[up_idx(i, j, p)] -= A[up_idx(k, i, p)] * A[up_idx(k, j, p)];
+            const int i = j1 + e / rem, j = j1 + e % rem;
+            if (j >= i) {
+                double v = 0.0;
+                for (int r = j0; r < j1; ++r) v += A[r * p + i] * A[r * p + j];
+                A[i * p + j] -= v;
+            }
         }
         __syncthreads();
     }
-    // X = R^{-1}, column c by back substitution over rows i = c .. 0 by a group of 8 lanes: lane q
-    // keeps X[j][c] for j = 8 jj + q in registers and the dot products are reduced across the group
-    constexpr int G = 8, JJ = kPMax / G;
-    const int q = tid % G, c = tid / G;
-    double xs[JJ];
-#pragma unroll
-    for (int jj = 0; jj < JJ; ++jj) xs[jj] = 0.0;
-    // every lane runs the same p steps (uniform shuffles); steps with i > c or c >= p do nothing
-    const bool col = c < p;
-    for (int i = p - 1; i >= 0; --i) {
-        double part = 0.0;
-#pragma unroll
-        for (int jj = 0; jj < JJ; ++jj) {
-            const int j = jj * G + q;
-            if (col && j > i && j <= c) part += A[up_idx(i, j, p)] * xs[jj];
-        }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, G);
-        if (col && i <= c && (i % G) == q) {
-            const double rii = A[up_idx(i, i, p)];
-            const double xic = (rii > 0.0) ? (((i == c) ? 1.0 : 0.0) - part) / rii : 0.0;
-#pragma unroll
-            for (int jj = 0; jj < JJ; ++jj)
-                if (jj == i / G) xs[jj] = xic;
+    // ---- R^{-1} in place: diagonal blocks first (into T, then back), then block columns left to right
+    for (int e = tid; e < nblk * kNB; e += nt) {
+        const int J = e / kNB, cl = e - J * kNB;
+        const int j0 = J * kNB, j1 = (j0 + kNB < p) ? j0 + kNB : p;
+        const int c = j0 + cl;
+        if (c < j1) {
+            for (int i = c; i >= j0; --i) {  // column c of the block's inverse, rows i <= c
+                double v = (i == c) ? 1.0 : 0.0;
+                for (int q = i + 1; q <= c; ++q) v -= A[i * p + q] * T[q * kNB + cl];
+                const double rii = A[i * p + i];
+                T[i * kNB + cl] = rii > 0.0 ? v / rii : 0.0;
+            }
         }
     }
-    // write column c of R^{-1} (rows j <= c from the registers, zero below the diagonal)
-    if (c < p) {
-#pragma unroll
-        for (int jj = 0; jj < JJ; ++jj) {
-            const int j = jj * G + q;
-            if (j < p) Rinv[(int64_t)j * p + c] = (j <= c) ? xs[jj] : 0.0;
+    __syncthreads();
+    for (int e = tid; e < p * kNB; e += nt) {
+        const int i = e / kNB, cl = e - i * kNB;
+        const int J = i / kNB, c = J * kNB + cl;
+        if (c < p && c >= i) A[i * p + c] = T[i * kNB + cl];
+    }
+    __syncthreads();
+    for (int J = 1; J < nblk; ++J) {
+        const int j0 = J * kNB, j1 = (j0 + kNB < p) ? j0 + kNB : p;
+        const int jb = j1 - j0;
+        // T = X[0:j0, 0:j0] R[0:j0, j0:j1] (X upper triangular, already in place)
+        for (int e = tid; e < j0 * jb; e += nt) {
+            const int i = e / jb, cl = e - i * jb;
+            double v = 0.0;
+            for (int q = i; q < j0; ++q) v += A[i * p + q] * A[q * p + j0 + cl];
+            T[i * kNB + cl] = v;
         }
+        __syncthreads();
+        // X[0:j0, j0:j1] = -T X[j0:j1, j0:j1]
+        for (int e = tid; e < j0 * jb; e += nt) {
+            const int i = e / jb, cl = e - i * jb;
+            double v = 0.0;
+            for (int q = 0; q <= cl; ++q) v += T[i * kNB + q] * A[(j0 + q) * p + j0 + cl];
+            A[i * p + j0 + cl] = -v;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < p * p; e += nt) {
+        const int i = e / p, j = e - i * p;
+        Rinv[e] = (j >= i) ? A[e] : 0.0;
     }
 }
 
@@ -772,7 +819,7 @@ static int orthonormalise(std::vector<Run> &runs, int passes, hipStream_t s) {
                 sb.in[i] = b.W;
                 sb.out[i] = b.R;
             },
-            k_chol_inv, sizeof(double) / 2, sizeof(double), proto, s);
+            k_chol_inv, sizeof(double), sizeof(double) * kNB, proto, s);
         if (rc) return rc;
         std::vector<gtn_gemm_desc> g;
         for (Run &r : runs) {

@@ -69,6 +69,8 @@ int lane();
 #define gridDim emu::gdim
 
 inline void __syncthreads() { emu::sync_block(); }
+// code-motion barrier among the lanes of one wavefront (on the CPU: a wave-wide rendezvous)
+inline void __builtin_amdgcn_wave_barrier() { emu::sync_wave(); }
 
 template <typename T>
 inline T __shfl_down(T v, unsigned delta, int width = 64) {
