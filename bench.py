@@ -6,10 +6,12 @@ exp(-0.5 * sum of leg popcounts), odd total parity zeroed, max|T| = 1, parity-pr
 statistics (1,1,-1,-1)) coarse-grained by Levin-Nave TRG for warmup + steps = 20 steps.  A step is
 one gtn.trg call (two signed permutations, two parity-block SVDs, six contractions, the norm).
 
-Prints one JSON line: value = steps/s of the mixed-precision path (contractions in split BF16 with
-FP32 accumulation, parity-block SVD / normalisation / log-norm sum in FP64), with the FP64 path's
-steps/s, ln Z per site of both against the FP64 CPU oracle, the roofline of the dominant kernel (the
-T' contraction GEMM) and the CPU baseline (the numpy oracle timed for one step on the host).
+Prints one JSON line: value = steps/s of the mixed-precision path (the first three steps all FP64,
+then contractions in split BF16 (three bf16 terms per operand) with FP32 accumulation and the SVD
+range finder in FP32; SVD projections, normalisation and the log-norm sum in FP64), with the FP64
+path's steps/s, ln Z per site of both against the FP64 CPU oracle, the roofline of the T' contraction
+GEMM (time from HIP events around each launch; HBM bytes from the committed rocprofv3 counter
+summary) and the CPU baseline (the numpy oracle timed for one step on the host).
 """
 import argparse
 import json
@@ -85,6 +87,20 @@ def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
     return {"wall_s": wall, "gpu_ms": gpu_ms, "lnz": complex(F), "tprime_ms": tp}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE passes scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "tprime_gemm_d64_chi64_hbm_bytes.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("machine-readable: "):
+            d = json.loads(line[len("machine-readable: "):])
+            if d.get("kernel") == kernel:
+                return d["traffic_bytes"]
+    return None
+
+
 def cpu_baseline():
     """One oracle TRG step at leg dim 64, chi 64 on the host (numpy + BLAS, all cores)."""
     from oracle import gtn_oracle as O
@@ -120,7 +136,7 @@ def main():
                     help="only run the copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--steps", type=int, default=18)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="bf16x3", help="contraction arithmetic of the mixed path")
+    ap.add_argument("--precision", default="bf16x6", help="contraction arithmetic of the mixed path")
     ap.add_argument("--svd-precision", default="f32", help="range-finder arithmetic of the mixed path")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="leading TRG steps of the mixed path run in FP64 (their normalisation errors "
@@ -183,7 +199,8 @@ def main():
                    "svd_orth_every": gtn.settings.svd_orth_every},
         "lnz_per_site_mixed": [mixed["lnz"].real, mixed["lnz"].imag],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": (achieved / peak) if achieved else None, "traffic": None,
+                     "frac": (achieved / peak) if achieved else None,
+                     "traffic": measured_traffic("tprime_gemm_" + args.precision),
                      "kernel": "T' contraction, 2 parity-block GEMMs of 2048^3, %s" % args.precision,
                      "flop_per_launch": flop, "avg_launch_ms": avg_ms, "launches": len(tp)},
     }
@@ -194,7 +211,13 @@ def main():
         out["dlnz_mixed_vs_fp64_gpu"] = abs(mixed["lnz"] - fp64["lnz"])
         if fp64["tprime_ms"]:
             ms64 = sum(fp64["tprime_ms"]) / len(fp64["tprime_ms"])
-            out["fp64_tprime_tflops"] = flop / (ms64 / 1000.0) / 1e12
+            a64 = flop / (ms64 / 1000.0) / 1e12
+            out["fp64_tprime_tflops"] = a64
+            out["roofline_fp64_path"] = {"bound": "mfma", "achieved": a64, "peak": MFMA_PEAK_TF["f64"],
+                                         "unit": "TFLOP/s", "frac": a64 / MFMA_PEAK_TF["f64"],
+                                         "traffic": measured_traffic("tprime_gemm_f64"),
+                                         "kernel": "T' contraction, 2 parity-block GEMMs of 2048^3, f64",
+                                         "avg_launch_ms": ms64, "launches": len(fp64["tprime_ms"])}
     if golden is not None:
         out["dlnz_mixed_vs_oracle"] = abs(mixed["lnz"] - golden)
         if fp64 is not None:
