@@ -163,7 +163,15 @@ class Contraction:
         return b
 
     def quadratic_form(self):
-        """(linear, quadratic) coefficients over GF(2) of the sign exponent, keyed by characters."""
+        """(linear, quadratic) coefficients over GF(2) of the sign exponent, keyed by characters
+        (computed once per contraction)."""
+        qf = getattr(self, "_qf", None)
+        if qf is not None:
+            return qf
+        self._qf = self._quadratic_form()
+        return self._qf
+
+    def _quadratic_form(self):
         ch = self.fermionic
         z = {c: 0 for c in ch}
         if self.sign_bit(z):
@@ -212,9 +220,37 @@ class Contraction:
         return parts
 
 
+_PLANS = {}
+
+
+def contraction(subscripts, operands):
+    """Contraction(subscripts, operands), shared between calls with the same subscripts, statistics and
+    shapes (a TRG step repeats the same handful of contractions every step)."""
+    key = (subscripts, tuple((tuple(st), tuple(shp)) for st, shp in operands))
+    plan = _PLANS.get(key)
+    if plan is None:
+        if len(_PLANS) > 4096:
+            _PLANS.clear()
+        plan = _PLANS[key] = Contraction(subscripts, operands)
+    return plan
+
+
+_TABLES = {}
+
+
 def sign_table(chars, lin, quad, zero_if_odd=False):
     """2-bit table over the parity bits of `chars` (bit i <-> chars[i]): 0 -> +1, 1 -> -1, 2 -> 0.
     zero_if_odd marks patterns of odd total parity as zero entries."""
+    key = (tuple(chars), tuple(sorted(lin.items())), tuple(sorted(quad.items())), bool(zero_if_odd))
+    codes = _TABLES.get(key)
+    if codes is None:
+        if len(_TABLES) > 4096:
+            _TABLES.clear()
+        codes = _TABLES[key] = _sign_table(chars, lin, quad, zero_if_odd)
+    return list(codes)
+
+
+def _sign_table(chars, lin, quad, zero_if_odd):
     n = len(chars)
     if n > 10:
         raise NotImplementedError("more than 10 fermionic indices on one tensor")

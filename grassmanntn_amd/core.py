@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib as L
 from ._device import DeviceArray, copy2d
 from ._perm import Perm, strides
-from ._plan import (BOSE, FERMI, HYBRID, SEPARATORS, Contraction, GrassmannError, denumerate, fail,
+from ._plan import (BOSE, FERMI, HYBRID, SEPARATORS, Contraction, GrassmannError, contraction, denumerate, fail,
                     fresh_char, group_info, parse_partition, sign_table)
 
 ENCODERS = ("canonical", "parity-preserving")
@@ -373,7 +373,7 @@ def einsum(*args, ignore_anticommutation=False):
     first = ops[0]
     if nobj >= 3:
         return _einsum_pairwise(subscripts, ops, ignore_anticommutation)
-    plan = Contraction(subscripts, [(o.statistics, o.shape) for o in ops])
+    plan = contraction(subscripts, [(o.statistics, o.shape) for o in ops])
     if nobj == 1:
         res = _einsum1(plan, ops[0], first, ignore_anticommutation)
     else:
@@ -873,10 +873,6 @@ def _block_prepare(Mdata, cutoff, rule="dense"):
     ceil(cutoff/2) even and floor(cutoff/2) odd with rule="block" (decompose_block,
     __init__.py:5076-5085)."""
     m, n = Mdata.shape
-    if not _pkg_flag("skip_parity_blocking_check", False):
-        leak = _parity_leak(Mdata)
-        if leak / (m * n / 2) > 1.0e-14:
-            fail("Error[BlockSVD]: This matrix is not constructed from a Grassmann-even tensor.")
     if cutoff is None:
         caps = (None, None)
     elif rule == "block":
@@ -935,15 +931,22 @@ def _interleave(stack, rows, cols, row_stride, par_stride):
     return P.run(stack)
 
 
-def _parity_leak(M):
-    """Frobenius norm of the parity-odd entries M[i, j], (i + j) odd, of a parity-preserving matrix.
-    (The reference sums |M_ij|, __init__.py:3977-3984; both are zero exactly for an even tensor.)"""
+def _parity_leak_dev(M):
+    """Squared Frobenius norm of the parity-odd entries M[i, j], (i + j) odd, of a parity-preserving
+    matrix, left on the device (read back with the decomposition's results, so that the check costs no
+    extra synchronisation).  The reference sums |M_ij| (__init__.py:3977-3984); both are zero exactly
+    for an even tensor."""
     lib = L.lib()
     dims = (C.c_int64 * 2)(*M.shape)
     one = (C.c_int32 * 2)(1, 1)
     out = DeviceArray.empty(1, np.float64)
     L.check(lib.gtn_odd_norm(L.ptr(M), 2, dims, one, one, L.dtype_code(M), L.ptr(out), L.stream_ptr()), "gtn_odd_norm")
-    return math.sqrt(float(out.item()))
+    return out
+
+
+def _check_parity_leak(out, m, n):
+    if math.sqrt(float(out.item())) / (m * n / 2) > 1.0e-14:
+        fail("Error[BlockSVD]: This matrix is not constructed from a Grassmann-even tensor.")
 
 
 def _strided_block(M, par, out=None):
@@ -1053,7 +1056,7 @@ def _svd_blocks_raw(blocks, mode):
 
 class _Job:
     """One svd / eig call in flight: the joined matrix, its blocks and what the split needs."""
-    __slots__ = ("T", "left", "right", "cutoff", "M", "blocks", "blocked", "kept", "done")
+    __slots__ = ("T", "left", "right", "cutoff", "M", "blocks", "blocked", "kept", "done", "leak")
 
 
 def _decompose_prepare(T, string, cutoff, mode, rule):
@@ -1066,7 +1069,7 @@ def _decompose_prepare(T, string, cutoff, mode, rule):
         if st not in ALLOWED_STAT:
             fail("Error[%s]: The input object contains illegal statistics. (0, 1, -1, or * only)" % name)
     job = _Job()
-    job.T, job.cutoff, job.kept, job.done = T, cutoff, [], None
+    job.T, job.cutoff, job.kept, job.done, job.leak = T, cutoff, [], None, None
     left, right = parse_partition(string, name)
     job.left, job.right = left, right
     fmt, enc = T.format, T.encoder
@@ -1111,6 +1114,8 @@ def _decompose_prepare(T, string, cutoff, mode, rule):
     else:
         job.blocked = True
         job.blocks = _block_prepare(Md, cutoff, rule)
+        if not _pkg_flag("skip_parity_blocking_check", False):
+            job.leak = (_parity_leak_dev(Md), Md.shape[0], Md.shape[1])
     return job
 
 
@@ -1175,6 +1180,9 @@ def _decompose_many(calls, mode, rule="dense"):
     jobs = [_decompose_prepare(T, string, cutoff, mode, rule) for T, string, cutoff in calls]
     specs = [b for j in jobs for b in j.blocks]
     outs, counts = _svd_blocks_raw(_materialize(specs), mode) if specs else ([], [])
+    for j in jobs:
+        if j.leak is not None:
+            _check_parity_leak(*j.leak)
     res, at = [], 0
     for j in jobs:
         nb = len(j.blocks)

@@ -13,6 +13,7 @@
 // The element type in HBM stays FP64 (or FP32); the reduced precision lives only between LDS and
 // the matrix cores.
 #include "gtn_common.h"
+#include <stdlib.h>
 #include <string.h>
 
 namespace gtn {
@@ -327,28 +328,62 @@ __global__ void __launch_bounds__(64) k_mfma_selftest(double *out) {
     }
 }
 
+// tile: 64 (64 x 64 tiles) or 128 (128 x 128 tiles; each of the 4 wavefronts owns 64 x 64)
 template <typename TIN>
-static int launch_gemm(GemmArgs &a, int precision, hipStream_t s) {
+static int launch_gemm(GemmArgs &a, int precision, int tile, hipStream_t s) {
     int total = a.tile_start[a.n_groups];
     if (total == 0) return GTN_OK;
     int grid = (total + 7) / 8 * 8;
+    const bool big = tile == 128;
     switch (precision) {
         case GTN_PREC_F64:
-            hipLaunchKernelGGL((k_gemm_fp<TIN, double, 64, 64, 16>), dim3(grid), dim3(256), 0, s, a);
+            if (big)
+                hipLaunchKernelGGL((k_gemm_fp<TIN, double, 128, 128, 16>), dim3(grid), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_gemm_fp<TIN, double, 64, 64, 16>), dim3(grid), dim3(256), 0, s, a);
             break;
         case GTN_PREC_F32:
-            hipLaunchKernelGGL((k_gemm_fp<TIN, float, 64, 64, 16>), dim3(grid), dim3(256), 0, s, a);
+            if (big)
+                hipLaunchKernelGGL((k_gemm_fp<TIN, float, 128, 128, 16>), dim3(grid), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_gemm_fp<TIN, float, 64, 64, 16>), dim3(grid), dim3(256), 0, s, a);
             break;
         case GTN_PREC_BF16X3:
-            hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+            if (big)
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
             break;
         case GTN_PREC_BF16X6:
-            hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+            if (big)
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
             break;
         default: set_error("gtn_gemm_grouped: unknown precision %d", precision); return GTN_ENOTSUP;
     }
     GTN_CHECK_LAUNCH();
     return GTN_OK;
+}
+
+// 128 x 128 tiles when every group is at least 128 x 128 and the launch still has a tile for every
+// compute unit; 64 x 64 otherwise.  GTN_GEMM_TILE=64|128 in the environment forces a size (128 only
+// where every group is at least 128 x 128) for tuning and tests.
+static int pick_tile(const gtn_gemm_desc *groups, int n) {
+    static int forced = -1;
+    if (forced < 0) {
+        const char *e = getenv("GTN_GEMM_TILE");
+        forced = e ? atoi(e) : 0;
+    }
+    int64_t t128 = 0;
+    for (int i = 0; i < n; ++i) {
+        const gtn_gemm_desc &d = groups[i];
+        if (d.M == 0 || d.N == 0 || d.batch == 0) continue;
+        if (d.M < 128 || d.N < 128) return 64;
+        t128 += ((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
+    }
+    if (forced == 64 || forced == 128) return forced;
+    return t128 >= 256 ? 128 : 64;
 }
 
 }  // namespace gtn
@@ -358,7 +393,8 @@ using namespace gtn;
 extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream) {
     GTN_REQUIRE(n_groups >= 0 && (n_groups == 0 || groups), "gtn_gemm_grouped: bad groups");
     GTN_REQUIRE(dtype == GTN_F64 || dtype == GTN_F32, "gtn_gemm_grouped: dtype must be F64 or F32");
-    const int BM = 64, BN = 64;
+    const int tile = pick_tile(groups, n_groups);
+    const int BM = tile, BN = tile;
     hipStream_t s = as_stream(stream);
     for (int base = 0; base < n_groups; base += kMaxGroups) {
         GemmArgs a;
@@ -422,7 +458,7 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
             }
             continue;
         }
-        int rc = dtype == GTN_F64 ? launch_gemm<double>(a, precision, s) : launch_gemm<float>(a, precision, s);
+        int rc = dtype == GTN_F64 ? launch_gemm<double>(a, precision, tile, s) : launch_gemm<float>(a, precision, tile, s);
         if (rc) return rc;
     }
     return GTN_OK;
