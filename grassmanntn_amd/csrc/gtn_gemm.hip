@@ -15,6 +15,9 @@
 #include "gtn_common.h"
 #include <stdlib.h>
 #include <string.h>
+#include <map>
+#include <mutex>
+#include <vector>
 
 namespace gtn {
 
@@ -24,13 +27,17 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMaxGroups = 12;
 
+// splits > 1: the reduction dimension is cut into chunks of kchunk; the tile enumeration runs over
+// batch * splits, and chunk sp of batch entry b writes its partial product to C + b * scb + sp * scs
+// (a workspace summed afterwards by k_split_sum).
 struct GemmGroup {
     const void *A;
     const void *B;
     void *C;
     int32_t M, N, K, batch;
     int32_t tiles_m, tiles_n;
-    int64_t sam, sak, sbk, sbn, scm, scn, sab, sbb, scb;
+    int32_t splits, kchunk;
+    int64_t sam, sak, sbk, sbn, scm, scn, sab, sbb, scb, scs;
     double alpha, beta;
 };
 
@@ -111,9 +118,13 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
     int gi, b, tm, tn;
     if (!locate(args, gi, b, tm, tn)) return;
     const GemmGroup &g = args.g[gi];
-    const TIN *A = (const TIN *)g.A + b * g.sab;
-    const TIN *B = (const TIN *)g.B + b * g.sbb;
-    TIN *C = (TIN *)g.C + b * g.scb;
+    const int sp = b % g.splits;
+    b /= g.splits;
+    const int kb = sp * g.kchunk;
+    const int K = min(g.K - kb, g.kchunk);
+    const TIN *A = (const TIN *)g.A + b * g.sab + kb * g.sak;
+    const TIN *B = (const TIN *)g.B + b * g.sbb + kb * g.sbk;
+    TIN *C = (TIN *)g.C + b * g.scb + sp * g.scs;
     constexpr int PAD = 2;
     __shared__ TC As[BK][BM + PAD];
     __shared__ TC Bs[BK][BN + PAD];
@@ -130,9 +141,9 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
         for (int j = 0; j < TJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     double ra[NA], rb[NB];
     const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
-    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, g.K, ra);
-    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, g.K, rb);
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
+    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
+    for (int k0 = 0; k0 < K; k0 += BK) {
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < NA; ++s) {
@@ -147,9 +158,9 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
             Bs[k][r] = (TC)rb[s];
         }
         __syncthreads();
-        if (k0 + BK < g.K) {  // prefetch the next panels while the matrix cores work
-            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, g.K, ra);
-            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, g.K, rb);
+        if (k0 + BK < K) {  // prefetch the next panels while the matrix cores work
+            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, K, ra);
+            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, K, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
@@ -199,9 +210,13 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
     int gi, b, tm, tn;
     if (!locate(args, gi, b, tm, tn)) return;
     const GemmGroup &g = args.g[gi];
-    const TIN *A = (const TIN *)g.A + b * g.sab;
-    const TIN *B = (const TIN *)g.B + b * g.sbb;
-    TIN *C = (TIN *)g.C + b * g.scb;
+    const int sp = b % g.splits;
+    b /= g.splits;
+    const int kb = sp * g.kchunk;
+    const int K = min(g.K - kb, g.kchunk);
+    const TIN *A = (const TIN *)g.A + b * g.sab + kb * g.sak;
+    const TIN *B = (const TIN *)g.B + b * g.sbb + kb * g.sbk;
+    TIN *C = (TIN *)g.C + b * g.scb + sp * g.scs;
     constexpr int PAD = 8;
     __shared__ __attribute__((aligned(16))) __bf16 As[NSPLIT][BM][BK + PAD];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[NSPLIT][BN][BK + PAD];
@@ -217,9 +232,9 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
         for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
     double ra[NA], rb[NB];
     const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
-    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, g.K, ra);
-    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, g.K, rb);
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
+    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
+    for (int k0 = 0; k0 < K; k0 += BK) {
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < NA; ++s) {
@@ -246,9 +261,9 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
             }
         }
         __syncthreads();
-        if (k0 + BK < g.K) {
-            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, g.K, ra);
-            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, g.K, rb);
+        if (k0 + BK < K) {
+            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, K, ra);
+            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, K, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 32) {
@@ -328,6 +343,53 @@ __global__ void __launch_bounds__(64) k_mfma_selftest(double *out) {
     }
 }
 
+// C[b](r, c) = alpha * sum_sp W[b][sp](r, c) + beta * C[b](r, c) over the split partials W
+// (batch x splits consecutive M x N matrices), in a fixed order.
+template <typename T>
+__global__ void k_split_sum(const T *__restrict__ W, int splits, int M, int N, int batch, T *C, int64_t scm,
+                            int64_t scn, int64_t scb, double alpha, double beta) {
+    const int64_t MN = (int64_t)M * N, total = MN * batch;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = i / MN, e = i - b * MN;
+        const int64_t r = e / N, c = e - r * N;
+        const T *w = W + b * splits * MN + e;
+        double acc = 0.0;
+        for (int sp = 0; sp < splits; ++sp) acc += (double)w[sp * MN];
+        const int64_t o = b * scb + r * scm + c * scn;
+        double v = alpha * acc;
+        if (beta != 0.0) v += beta * (double)C[o];
+        C[o] = (T)v;
+    }
+}
+
+// Split-K workspace: one grow-only buffer per device.  Launches on one stream are ordered; a launch
+// on another stream first waits for the event recorded after the buffer's last use.
+struct SplitWs {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipStream_t last = nullptr;
+    hipEvent_t done = nullptr;
+    bool used = false;
+};
+static std::mutex g_ws_mu;
+static std::map<int, SplitWs> g_ws;
+
+static int split_workspace(SplitWs &w, size_t bytes, hipStream_t s) {
+    if (w.used && w.last != s) GTN_CHECK_HIP(hipStreamWaitEvent(s, w.done, 0));
+    if (w.bytes < bytes) {
+        if (w.p) {
+            GTN_CHECK_HIP(hipDeviceSynchronize());
+            GTN_CHECK_HIP(hipFree(w.p));
+            w.p = nullptr;
+            w.bytes = 0;
+        }
+        GTN_CHECK_HIP(hipMalloc(&w.p, bytes));
+        w.bytes = bytes;
+    }
+    if (!w.done) GTN_CHECK_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+    return GTN_OK;
+}
+
 // tile: 64 (64 x 64 tiles) or 128 (128 x 128 tiles; each of the 4 wavefronts owns 64 x 64)
 template <typename TIN>
 static int launch_gemm(GemmArgs &a, int precision, int tile, hipStream_t s) {
@@ -386,6 +448,16 @@ static int pick_tile(const gtn_gemm_desc *groups, int n) {
     return t128 >= 256 ? 128 : 64;
 }
 
+// GTN_GEMM_SPLIT=0 in the environment turns the split reduction off (tuning and tests).
+static bool split_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("GTN_GEMM_SPLIT");
+        on = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    return on == 1;
+}
+
 }  // namespace gtn
 
 using namespace gtn;
@@ -435,6 +507,9 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
             g.scb = d.scb;
             g.alpha = d.alpha;
             g.beta = d.beta;
+            g.splits = 1;
+            g.kchunk = g.K;
+            g.scs = 0;
             a.tile_start[used] = tiles;
             int64_t t = (int64_t)g.tiles_m * g.tiles_n * g.batch;
             GTN_REQUIRE(tiles + t < (1ll << 31), "gtn_gemm_grouped: too many tiles");
@@ -458,8 +533,78 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
             }
             continue;
         }
+        // Too few tiles to give every compute unit two workgroups: cut long reductions into chunks
+        // (partials to the workspace, summed in a fixed order afterwards) until the launch has about
+        // four workgroups per compute unit.  A chunk keeps at least 256 of the reduction.
+        const int es = dtype == GTN_F64 ? 8 : 4;
+        std::vector<GemmGroup> orig;
+        std::unique_lock<std::mutex> lk(g_ws_mu, std::defer_lock);
+        SplitWs *ws = nullptr;
+        if (tiles < 512 && split_enabled()) {
+            int S = 1;
+            while (tiles * S < 1024 && S < 16) S *= 2;
+            int64_t need = 0;
+            int splitting = 0;
+            for (int i = 0; i < used; ++i) {
+                GemmGroup &g = a.g[i];
+                int Sg = S;
+                while (Sg > 1 && g.K / Sg < 256) Sg /= 2;
+                if (Sg > 1) {
+                    const int chunk = (g.K / Sg + 31) / 32 * 32;
+                    Sg = (g.K + chunk - 1) / chunk;
+                    g.splits = Sg;
+                    g.kchunk = chunk;
+                    need += (int64_t)Sg * g.batch * g.M * g.N;
+                    ++splitting;
+                }
+            }
+            if (splitting) {
+                int dev;
+                GTN_CHECK_HIP(hipGetDevice(&dev));
+                lk.lock();
+                ws = &g_ws[dev];
+                int rc = split_workspace(*ws, (size_t)need * es, s);
+                if (rc) return rc;
+                orig.assign(a.g, a.g + used);
+                int64_t off = 0;
+                tiles = 0;
+                for (int i = 0; i < used; ++i) {
+                    GemmGroup &g = a.g[i];
+                    a.tile_start[i] = tiles;
+                    if (g.splits > 1) {
+                        g.C = (char *)ws->p + off * es;
+                        g.scm = g.N;
+                        g.scn = 1;
+                        g.scs = (int64_t)g.M * g.N;
+                        g.scb = g.scs * g.splits;
+                        g.alpha = 1.0;
+                        g.beta = 0.0;
+                        off += g.scb * g.batch;
+                    }
+                    tiles += g.tiles_m * g.tiles_n * g.batch * g.splits;
+                }
+                a.tile_start[used] = tiles;
+            }
+        }
         int rc = dtype == GTN_F64 ? launch_gemm<double>(a, precision, tile, s) : launch_gemm<float>(a, precision, tile, s);
         if (rc) return rc;
+        if (ws) {
+            for (int i = 0; i < used; ++i) {
+                const GemmGroup &g = a.g[i], &o = orig[i];
+                if (g.splits == 1) continue;
+                const dim3 grid(grid_for((int64_t)g.M * g.N * g.batch, 256, 4096)), blk(256);
+                if (dtype == GTN_F64)
+                    hipLaunchKernelGGL(k_split_sum<double>, grid, blk, 0, s, (const double *)g.C, g.splits, g.M, g.N,
+                                       g.batch, (double *)o.C, o.scm, o.scn, o.scb, o.alpha, o.beta);
+                else
+                    hipLaunchKernelGGL(k_split_sum<float>, grid, blk, 0, s, (const float *)g.C, g.splits, g.M, g.N,
+                                       g.batch, (float *)o.C, o.scm, o.scn, o.scb, o.alpha, o.beta);
+                GTN_CHECK_LAUNCH();
+            }
+            GTN_CHECK_HIP(hipEventRecord(ws->done, s));
+            ws->last = s;
+            ws->used = true;
+        }
     }
     return GTN_OK;
 }

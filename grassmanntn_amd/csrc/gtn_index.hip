@@ -1,10 +1,14 @@
 // Integer index maps and sign factors on the device (param.py:61-83; __init__.py:1011-1154,
 // 1944-2126, 2947-3170), and the FP64 reductions / elementwise helpers of the TRG step.
 #include "gtn_common.h"
+#include <stdlib.h>
 #include <string.h>
 #include <math.h>
 #include <vector>
 #include <utility>
+#include <map>
+#include <mutex>
+#include <string>
 
 namespace gtn {
 
@@ -66,31 +70,24 @@ struct PermArgs {
     int64_t total;
 };
 
-// Returns the source offset of output element `lin` and its table entry (0: +1, 1: -1, 2: zero).
-__device__ __forceinline__ int64_t perm_map(const PermArgs &a, uint64_t lin, uint32_t &code) {
-    uint64_t V[AX];
+// Contribution of the virtual axes in `vmask` for virtual-axis values V: their components' parity
+// bits and sign flips, and the offset and sign flips of the input axes in `inmask`.  With every
+// axis in the masks this is the whole map; a subset is exact when the inputs in `inmask` are fed
+// only by components of the virtual axes in `vmask` (inputs fed by nobody hold index 0, whose
+// offset, encoding and sigma are all trivial).
+__device__ __forceinline__ void perm_core(const PermArgs &a, const uint64_t (&V)[AX], uint32_t vmask,
+                                          uint32_t inmask, int64_t &off, uint32_t &bits, uint32_t &neg) {
     uint64_t F[AX];
 #pragma unroll
-    for (int b = 0; b < AX; ++b) {
-        V[b] = 0;
-        F[b] = 0;
-    }
-    uint64_t rest = lin;
-    for (int ax = a.n_out - 1; ax >= 0; --ax) {
-        uint64_t d = a.out_dim[ax], o;
-        if (a.out_shift[ax] >= 0) {
-            o = rest & (d - 1);
-            rest >>= a.out_shift[ax];
-        } else {
-            o = rest % d;
-            rest /= d;
-        }
-        if (a.out_virt[ax][0] >= 0) V[a.out_virt[ax][0]] += o * (uint64_t)a.out_vmul[ax][0];
-        if (a.out_virt[ax][1] >= 0) V[a.out_virt[ax][1]] += o * (uint64_t)a.out_vmul[ax][1];
-    }
-    uint32_t bits = 0, neg = 0;
+    for (int b = 0; b < AX; ++b) F[b] = 0;
+    bits = 0;
+    neg = 0;
     int j = a.n_comp;
     for (int v = a.n_virt - 1; v >= 0; --v) {
+        if (!((vmask >> v) & 1u)) {
+            j -= a.virt_ncomp[v];
+            continue;
+        }
         uint64_t c = ((a.virt_enc_mask >> v) & 1) ? enc_switch(V[v]) : V[v];
         if ((a.virt_sigma_mask >> v) & 1) neg ^= sigma_bit(c);
         for (int t = 0; t < a.virt_ncomp[v]; ++t) {
@@ -110,8 +107,9 @@ __device__ __forceinline__ int64_t perm_map(const PermArgs &a, uint64_t lin, uin
             if (a.comp_in[j][1] >= 0) F[a.comp_in[j][1]] += k * (uint64_t)a.comp_mult[j][1];
         }
     }
-    int64_t off = 0;
+    off = 0;
     for (int b = 0; b < a.n_in; ++b) {
+        if (!((inmask >> b) & 1u)) continue;
         uint64_t f = F[b];
         if ((a.in_sigma_mask >> b) & 1) neg ^= sigma_bit(f);
         uint64_t e = ((a.in_enc_mask >> b) & 1) ? enc_switch(f) : f;
@@ -120,17 +118,135 @@ __device__ __forceinline__ int64_t perm_map(const PermArgs &a, uint64_t lin, uin
         else
             off += (int64_t)e * a.in_stride[b];
     }
+}
+
+__device__ __forceinline__ uint32_t sign_code(const PermArgs &a, uint32_t bits, uint32_t neg) {
     uint32_t t = (a.table[bits >> 4] >> ((bits & 15u) * 2)) & 3u;
-    code = (t == 2u) ? 2u : (t ^ neg);
+    return (t == 2u) ? 2u : (t ^ neg);
+}
+
+// Returns the source offset of output element `lin` and its table entry (0: +1, 1: -1, 2: zero).
+__device__ __forceinline__ int64_t perm_map(const PermArgs &a, uint64_t lin, uint32_t &code) {
+    uint64_t V[AX];
+#pragma unroll
+    for (int b = 0; b < AX; ++b) V[b] = 0;
+    uint64_t rest = lin;
+    for (int ax = a.n_out - 1; ax >= 0; --ax) {
+        uint64_t d = a.out_dim[ax], o;
+        if (a.out_shift[ax] >= 0) {
+            o = rest & (d - 1);
+            rest >>= a.out_shift[ax];
+        } else {
+            o = rest % d;
+            rest /= d;
+        }
+        if (a.out_virt[ax][0] >= 0) V[a.out_virt[ax][0]] += o * (uint64_t)a.out_vmul[ax][0];
+        if (a.out_virt[ax][1] >= 0) V[a.out_virt[ax][1]] += o * (uint64_t)a.out_vmul[ax][1];
+    }
+    int64_t off;
+    uint32_t bits, neg;
+    perm_core(a, V, 0xffffffffu, 0xffffffffu, off, bits, neg);
+    code = sign_code(a, bits, neg);
     return off;
 }
 
-template <typename T>
-__global__ void k_permute(PermArgs a, const T *__restrict__ src, T *__restrict__ dst) {
+// ---------------------------------------------------------------------------------------------
+// Table-driven map.  The virtual axes fall into groups: two virtual axes share a group when their
+// components feed a common input axis.  A group's contribution (source offset, parity bits, sign
+// flip) depends only on the values of its own virtual axes, so it is tabulated once per gather
+// shape, indexed by G = sum over the group's axes of V_v * radix_v, itself a linear function of the
+// output indices.  Per element the map is then an index decomposition, at most kTabGroups table
+// reads, a sum and an XOR, instead of the component walk of perm_map.
+// ---------------------------------------------------------------------------------------------
+constexpr int kTabGroups = 4;
+struct alignas(16) TabEntry {
+    int64_t off;
+    uint32_t code;  // parity bits | sign flip << 31
+    uint32_t pad;
+};
+struct TabArgs {
+    int32_t ng;
+    uint32_t gmul[AX][kTabGroups];  // G_g = sum_ax o_ax * gmul[ax][g]
+    const TabEntry *tab[kTabGroups];
+};
+struct TabBuild {
+    int32_t ng;
+    int64_t start[kTabGroups + 1];  // first entry of each group's table
+    uint32_t vmask[kTabGroups], inmask[kTabGroups];
+    int32_t nv[kTabGroups];
+    int8_t vid[kTabGroups][AX];
+    uint32_t vrad[kTabGroups][AX], vext[kTabGroups][AX];
+};
+
+__global__ void k_perm_tables(PermArgs a, TabBuild t, TabEntry *out) {
+    const int64_t n = t.start[t.ng];
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        int g = 0;
+        while (g + 1 < t.ng && e >= t.start[g + 1]) ++g;
+        const uint32_t G = (uint32_t)(e - t.start[g]);
+        uint64_t V[AX];
+#pragma unroll
+        for (int b = 0; b < AX; ++b) V[b] = 0;
+        for (int i = 0; i < t.nv[g]; ++i) V[t.vid[g][i]] = (G / t.vrad[g][i]) % t.vext[g][i];
+        int64_t off;
+        uint32_t bits, neg;
+        perm_core(a, V, t.vmask[g], t.inmask[g], off, bits, neg);
+        TabEntry r;
+        r.off = off;
+        r.code = bits | (neg << 31);
+        r.pad = 0;
+        out[e] = r;
+    }
+}
+
+__device__ __forceinline__ int64_t tab_map(const PermArgs &a, const TabArgs &t, uint64_t lin, uint32_t &code) {
+    uint32_t G[kTabGroups];
+#pragma unroll
+    for (int g = 0; g < kTabGroups; ++g) G[g] = 0;
+    uint64_t rest = lin;
+    for (int ax = a.n_out - 1; ax >= 0; --ax) {
+        uint32_t o;
+        if (a.out_shift[ax] >= 0) {
+            o = (uint32_t)(rest & (uint64_t)(a.out_dim[ax] - 1));
+            rest >>= a.out_shift[ax];
+        } else if ((rest >> 32) == 0) {
+            const uint32_t r32 = (uint32_t)rest, d = a.out_dim[ax];
+            o = r32 % d;
+            rest = r32 / d;
+        } else {
+            o = (uint32_t)(rest % a.out_dim[ax]);
+            rest /= a.out_dim[ax];
+        }
+#pragma unroll
+        for (int g = 0; g < kTabGroups; ++g) G[g] += o * t.gmul[ax][g];
+    }
+    int64_t off = 0;
+    uint32_t c = 0;
+#pragma unroll
+    for (int g = 0; g < kTabGroups; ++g)
+        if (g < t.ng) {
+            const TabEntry e = t.tab[g][G[g]];
+            off += e.off;
+            c ^= e.code;
+        }
+    code = sign_code(a, c & 0x7fffffffu, c >> 31);
+    return off;
+}
+
+template <bool TAB>
+__device__ __forceinline__ int64_t map_elem(const PermArgs &a, const TabArgs &t, uint64_t lin, uint32_t &code) {
+    if constexpr (TAB)
+        return tab_map(a, t, lin, code);
+    else
+        return perm_map(a, lin, code);
+}
+
+template <typename T, bool TAB>
+__global__ void k_permute(PermArgs a, TabArgs tb, const T *__restrict__ src, T *__restrict__ dst) {
     const double al = a.alpha_re;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.total; i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t code;
-        int64_t off = perm_map(a, (uint64_t)i, code);
+        int64_t off = map_elem<TAB>(a, tb, (uint64_t)i, code);
         double v = (code == 2u) ? 0.0 : (double)src[off] * (code ? -al : al);
         dst[i] = (T)v;
     }
@@ -162,8 +278,8 @@ struct TileArgs {
     int64_t rest_total;
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_permute_tiled(PermArgs a, TileArgs t, const T *__restrict__ src,
+template <typename T, bool TAB>
+__global__ void __launch_bounds__(256) k_permute_tiled(PermArgs a, TabArgs tb, TileArgs t, const T *__restrict__ src,
                                                        T *__restrict__ dst) {
     __shared__ double tile[64][65];
     const double al = a.alpha_re;
@@ -188,7 +304,7 @@ __global__ void __launch_bounds__(256) k_permute_tiled(PermArgs a, TileArgs t, c
             double v = 0.0;
             if (i1 < d1 && i0 < d0) {
                 uint32_t code;
-                const int64_t off = perm_map(a, (uint64_t)(base + i1 * s1 + i0), code);
+                const int64_t off = map_elem<TAB>(a, tb, (uint64_t)(base + i1 * s1 + i0), code);
                 v = (code == 2u) ? 0.0 : (double)src[off] * (code ? -al : al);
             }
             tile[l1][l0] = v;
@@ -563,6 +679,157 @@ static bool plan_tiles(const gtn_perm_desc *d, TileArgs &t) {
     return true;
 }
 
+// Group the virtual axes for the table-driven map (k_perm_tables).  Returns false when the gather
+// is too small to repay a table or a group's table would be large.
+static bool plan_groups(const gtn_perm_desc *d, int64_t total, TabBuild &tb, TabArgs &ta) {
+    if (total < (1 << 16) || d->n_virt == 0) return false;
+    const int nv = d->n_virt;
+    int64_t ext[AX];
+    for (int v = 0; v < nv; ++v) ext[v] = 1;
+    for (int ax = 0; ax < d->n_out; ++ax)
+        for (int t = 0; t < 2; ++t)
+            if (d->out_virt[ax][t] >= 0 && d->out_dim[ax] > 0)
+                ext[d->out_virt[ax][t]] += (d->out_dim[ax] - 1) * d->out_vmul[ax][t];
+    // union virtual axes whose components feed a common input axis
+    int parent[AX];
+    for (int v = 0; v < nv; ++v) parent[v] = v;
+    auto root = [&](int v) {
+        while (parent[v] != v) v = parent[v];
+        return v;
+    };
+    int owner[AX];
+    uint32_t feeds[AX] = {0};  // input axes fed by each virtual axis
+    for (int b = 0; b < d->n_in; ++b) owner[b] = -1;
+    int j = 0;
+    for (int v = 0; v < nv; ++v)
+        for (int t = 0; t < d->virt_ncomp[v]; ++t, ++j)
+            for (int u = 0; u < 2; ++u) {
+                const int b = d->comp_in[j][u];
+                if (b < 0) continue;
+                feeds[v] |= 1u << b;
+                if (owner[b] < 0)
+                    owner[b] = v;
+                else
+                    parent[root(v)] = root(owner[b]);
+            }
+    // groups in order of their smallest virtual axis
+    int gid[AX], ng = 0;
+    int64_t gsize[AX];
+    int members[AX][AX], nmem[AX];
+    for (int v = 0; v < nv; ++v) gid[v] = -1;
+    for (int v = 0; v < nv; ++v) {
+        const int r = root(v);
+        if (gid[r] < 0) {
+            gid[r] = ng;
+            gsize[ng] = 1;
+            nmem[ng] = 0;
+            ++ng;
+        }
+        const int g = gid[r];
+        members[g][nmem[g]++] = v;
+        gsize[g] *= ext[v];
+        if (gsize[g] > (1 << 22)) return false;
+    }
+    // merge the two smallest groups while there are too many
+    while (ng > kTabGroups) {
+        int a0 = 0, a1 = 1;
+        if (gsize[a1] < gsize[a0]) std::swap(a0, a1);
+        for (int g = 2; g < ng; ++g) {
+            if (gsize[g] < gsize[a0]) {
+                a1 = a0;
+                a0 = g;
+            } else if (gsize[g] < gsize[a1]) {
+                a1 = g;
+            }
+        }
+        if (a1 < a0) std::swap(a0, a1);
+        gsize[a0] *= gsize[a1];
+        if (gsize[a0] > (1 << 22)) return false;
+        for (int i = 0; i < nmem[a1]; ++i) members[a0][nmem[a0]++] = members[a1][i];
+        for (int g = a1; g + 1 < ng; ++g) {
+            gsize[g] = gsize[g + 1];
+            nmem[g] = nmem[g + 1];
+            for (int i = 0; i < nmem[g]; ++i) members[g][i] = members[g + 1][i];
+        }
+        --ng;
+    }
+    int64_t entries = 0;
+    for (int g = 0; g < ng; ++g) entries += gsize[g];
+    if (entries * 4 > total) return false;
+    memset(&tb, 0, sizeof(tb));
+    memset(&ta, 0, sizeof(ta));
+    tb.ng = ta.ng = ng;
+    int64_t rad_of[AX];
+    int grp_of[AX];
+    for (int g = 0; g < ng; ++g) {
+        tb.start[g + 1] = tb.start[g] + gsize[g];
+        tb.nv[g] = nmem[g];
+        int64_t rad = 1;
+        for (int i = nmem[g] - 1; i >= 0; --i) {
+            const int v = members[g][i];
+            tb.vid[g][i] = (int8_t)v;
+            tb.vrad[g][i] = (uint32_t)rad;
+            tb.vext[g][i] = (uint32_t)ext[v];
+            rad_of[v] = rad;
+            grp_of[v] = g;
+            rad *= ext[v];
+            tb.vmask[g] |= 1u << v;
+            tb.inmask[g] |= feeds[v];
+        }
+    }
+    for (int ax = 0; ax < d->n_out; ++ax)
+        for (int t = 0; t < 2; ++t) {
+            const int v = d->out_virt[ax][t];
+            if (v < 0) continue;
+            ta.gmul[ax][grp_of[v]] += (uint32_t)(d->out_vmul[ax][t] * rad_of[v]);
+        }
+    return true;
+}
+
+// Device tables of the table-driven map, built once per gather shape and device and kept until
+// the cache outgrows kTabCacheBytes (then every table is released after a device synchronisation).
+struct TabCacheEntry {
+    TabEntry *dev;
+    int64_t entries;
+};
+static std::mutex g_tab_mu;
+static std::map<std::string, TabCacheEntry> g_tab_cache;
+static int64_t g_tab_bytes = 0;
+constexpr int64_t kTabCacheBytes = 64ll << 20;
+
+static int tab_tables(const PermArgs &a, const TabBuild &tb, hipStream_t s, TabArgs &ta) {
+    int dev;
+    GTN_CHECK_HIP(hipGetDevice(&dev));
+    PermArgs k;
+    memcpy(&k, &a, sizeof(k));
+    memset(k.table, 0, sizeof(k.table));
+    k.alpha_re = k.alpha_im = 0.0;
+    std::string key((const char *)&dev, sizeof(dev));
+    key.append((const char *)&k, sizeof(k));
+    key.append((const char *)&tb, sizeof(tb));
+    std::lock_guard<std::mutex> l(g_tab_mu);
+    auto it = g_tab_cache.find(key);
+    if (it == g_tab_cache.end()) {
+        const int64_t n = tb.start[tb.ng];
+        if (g_tab_bytes + n * (int64_t)sizeof(TabEntry) > kTabCacheBytes && !g_tab_cache.empty()) {
+            GTN_CHECK_HIP(hipDeviceSynchronize());
+            for (auto &e : g_tab_cache) GTN_CHECK_HIP(hipFree(e.second.dev));
+            g_tab_cache.clear();
+            g_tab_bytes = 0;
+        }
+        TabEntry *p = nullptr;
+        GTN_CHECK_HIP(hipMalloc(&p, n * sizeof(TabEntry)));
+        hipLaunchKernelGGL(k_perm_tables, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, tb, p);
+        GTN_CHECK_LAUNCH();
+        // the tables are shared by later gathers on any stream: finish them here
+        GTN_CHECK_HIP(hipStreamSynchronize(s));
+        it = g_tab_cache.emplace(key, TabCacheEntry{p, n}).first;
+        g_tab_bytes += n * (int64_t)sizeof(TabEntry);
+    }
+    for (int g = 0; g < tb.ng; ++g) ta.tab[g] = it->second.dev + tb.start[g];
+    return GTN_OK;
+}
+
 int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream) {
     GTN_REQUIRE(d && src && dst, "gtn_permute: null argument");
     PermArgs a;
@@ -576,21 +843,46 @@ int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream
         return GTN_OK;
     }
     hipStream_t s = as_stream(stream);
+    TabArgs tb;
+    memset(&tb, 0, sizeof(tb));
+    TabBuild build;
+    const bool tab = d->dtype != GTN_C128 && getenv("GTN_PERM_NO_TABLES") == nullptr && plan_groups(d, a.total, build, tb);
+    if (tab) {
+        rc = tab_tables(a, build, s, tb);
+        if (rc) return rc;
+    }
     TileArgs t;
     if (d->dtype != GTN_C128 && plan_tiles(d, t)) {
         const int64_t nblk = (int64_t)t.tiles0 * t.tiles1 * t.rest_total;
         dim3 gt((unsigned)(nblk < (1 << 20) ? nblk : (1 << 20))), bt(256);
-        if (d->dtype == GTN_F64)
-            hipLaunchKernelGGL(k_permute_tiled<double>, gt, bt, 0, s, a, t, (const double *)src, (double *)dst);
-        else
-            hipLaunchKernelGGL(k_permute_tiled<float>, gt, bt, 0, s, a, t, (const float *)src, (float *)dst);
+        if (d->dtype == GTN_F64) {
+            if (tab)
+                hipLaunchKernelGGL((k_permute_tiled<double, true>), gt, bt, 0, s, a, tb, t, (const double *)src, (double *)dst);
+            else
+                hipLaunchKernelGGL((k_permute_tiled<double, false>), gt, bt, 0, s, a, tb, t, (const double *)src, (double *)dst);
+        } else {
+            if (tab)
+                hipLaunchKernelGGL((k_permute_tiled<float, true>), gt, bt, 0, s, a, tb, t, (const float *)src, (float *)dst);
+            else
+                hipLaunchKernelGGL((k_permute_tiled<float, false>), gt, bt, 0, s, a, tb, t, (const float *)src, (float *)dst);
+        }
         GTN_CHECK_LAUNCH();
         return GTN_OK;
     }
     dim3 g(grid_for(a.total, 256, 1 << 20)), b(256);
     switch (d->dtype) {
-        case GTN_F64: hipLaunchKernelGGL(k_permute<double>, g, b, 0, s, a, (const double *)src, (double *)dst); break;
-        case GTN_F32: hipLaunchKernelGGL(k_permute<float>, g, b, 0, s, a, (const float *)src, (float *)dst); break;
+        case GTN_F64:
+            if (tab)
+                hipLaunchKernelGGL((k_permute<double, true>), g, b, 0, s, a, tb, (const double *)src, (double *)dst);
+            else
+                hipLaunchKernelGGL((k_permute<double, false>), g, b, 0, s, a, tb, (const double *)src, (double *)dst);
+            break;
+        case GTN_F32:
+            if (tab)
+                hipLaunchKernelGGL((k_permute<float, true>), g, b, 0, s, a, tb, (const float *)src, (float *)dst);
+            else
+                hipLaunchKernelGGL((k_permute<float, false>), g, b, 0, s, a, tb, (const float *)src, (float *)dst);
+            break;
         case GTN_C128: hipLaunchKernelGGL(k_permute_c128, g, b, 0, s, a, (const double2 *)src, (double2 *)dst); break;
         default: set_error("gtn_permute: unsupported dtype %d", d->dtype); return GTN_ENOTSUP;
     }

@@ -1,4 +1,5 @@
-"""Matrix-core lane layouts the GEMM kernels assume, checked on the device (gtn_mfma_selftest)."""
+"""Matrix-core lane layouts the GEMM kernels assume, checked on the device (gtn_mfma_selftest), and the
+split-reduction path of gtn_gemm_grouped against numpy."""
 import numpy as np
 import pytest
 
@@ -32,3 +33,42 @@ def test_mfma_lane_layout(form, name):
         print(name, "observed rows per (lane, v):", dec[:32])
         print(name, "observed values:", got[:32])
     np.testing.assert_array_equal(got, exp, err_msg=name)
+
+
+def _gemm_case(rng, M, N, K, batch, a_rowmajor, scm_pad, alpha, beta):
+    """Host operands of one gtn_gemm_grouped group: A (batch, M, K) stored row-major or transposed,
+    B (batch, K, N), C (batch, M, N + scm_pad) holding values that beta scales."""
+    A = rng.standard_normal((batch, M, K)) if a_rowmajor else rng.standard_normal((batch, K, M))
+    B = rng.standard_normal((batch, K, N))
+    C = rng.standard_normal((batch, M, N + scm_pad))
+    Aop = A if a_rowmajor else A.transpose(0, 2, 1)
+    ref = C.copy()
+    ref[:, :, :N] = alpha * (Aop @ B) + beta * C[:, :, :N]
+    return A, B, C, ref
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5), ("bf16x6", 1e-5)])
+def test_gemm_split_reduction(prec, tol):
+    """Launches with few tiles and long reductions are cut into K chunks (ragged last chunk) whose
+    partial products are summed afterwards; checked with alpha, beta, batch strides, a padded C and
+    a transposed A against numpy."""
+    import ctypes as C
+    from grassmanntn_amd import _lib as L
+    from grassmanntn_amd.core import _gemm_desc, _gemm
+    from grassmanntn_amd._device import DeviceArray
+    rng = np.random.default_rng(11)
+    cases = [(192, 80, 3000, 3, True, 5, 0.5, -2.0), (100, 64, 1000, 1, False, 0, 1.0, 0.0)]
+    groups, keep = [], []
+    for (M, N, K, batch, rowm, pad, alpha, beta) in cases:
+        A, B, Cm, ref = _gemm_case(rng, M, N, K, batch, rowm, pad, alpha, beta)
+        dA, dB, dC = DeviceArray.from_numpy(A), DeviceArray.from_numpy(B), DeviceArray.from_numpy(Cm)
+        sam, sak = (K, 1) if rowm else (1, M)
+        groups.append(_gemm_desc(dA.data_ptr(), sam, sak, dB.data_ptr(), N, 1, dC.data_ptr(), N + pad, 1, M, N, K,
+                                 batch=batch, sab=M * K, sbb=K * N, scb=M * (N + pad), alpha=alpha, beta=beta))
+        keep.append((dA, dB, dC, ref, N))
+    _gemm(groups, L.dtype_code(keep[0][0]), prec)
+    for dA, dB, dC, ref, N in keep:
+        got = dC.numpy()
+        scale = np.abs(ref).max()
+        np.testing.assert_allclose(got[:, :, N:], ref[:, :, N:], rtol=0, atol=0)  # padding untouched
+        assert np.abs(got - ref).max() <= tol * scale, (prec, np.abs(got - ref).max() / scale)
