@@ -151,136 +151,188 @@ __device__ double wg_sum(double v, double *sh) {
 }
 
 // Cholesky W = R^T R (W p x p symmetric positive semi-definite, shifted by shift_rel * trace / p),
-// then out = R^{-1} (full p x p, zero below the diagonal).  Both run blocked in 16-wide panels on the
-// full matrix in LDS (p^2 + 16 p doubles): a wavefront factors each diagonal block, every thread
-// solves one column of the panel rows, the trailing update is a 16-deep rank update; the inverse
-// inverts the diagonal blocks and then fills block columns left to right (two barriers each).  A
-// pivot at or below 1e-14 of the largest diagonal entry marks its column as dependent: its row of R
-// and its row and column of R^{-1} are zero, so that a rank-deficient basis gives zero columns
-// instead of overflowing ones.
-__device__ __forceinline__ int up_idx(int i, int j, int p) { return i * p - (i * (i - 1)) / 2 + (j - i); }
+// then out = R^{-1} (full p x p, zero below the diagonal).  The matrix is held in LDS padded to
+// N = 16 * ceil(p / 16) (row stride N + 1, so that row and column walks of 16 lanes hit distinct
+// banks; the padding is zero and factors as dependent columns) and processed in 16 x 16 tiles:
+//   - one wavefront factors each diagonal tile in registers (lane c holds column c, broadcasts by
+//     lane reads) and inverts it (Dinv_J, kept beside the matrix);
+//   - the panel right of it is X = Dinv_J^T A[J, >J] and the trailing update A[I, T] -= X_I^T X_T,
+//     one FP64 16x16x4 MFMA tile per wavefront;
+//   - R^{-1} fills block columns left to right: X[<J, J] = -(X[<J, <J] R[<J, J]) Dinv_J, the product
+//     staged in the (unused) lower block triangle.
+// A pivot at or below 1e-14 of the largest diagonal entry (or NaN) marks its column as dependent:
+// its row of R and its row and column of R^{-1} are zero, so that a rank-deficient basis gives
+// zero columns instead of overflowing ones.
+constexpr int kNB = 16;  // tile width of the blocked Cholesky / triangular inverse
 
-constexpr int kNB = 16;  // panel width of the blocked Cholesky / triangular inverse
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// value of `v` in lane `l` of the wavefront (l uniform)
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Factor the 16 x 16 diagonal tile at T (row stride ld) in place (upper triangle R_JJ, zero below)
+// and write its inverse to D (16 x 16, row stride 16, zero below).  Called by a whole wavefront;
+// lanes 16..63 compute alongside and store nothing.
+__device__ __forceinline__ void chol_tile(double *T, int ld, double *D, double tol, int lane) {
+    const int c = lane;
+    double col[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) col[r] = (c < 16 && r <= c) ? T[r * ld + c] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double d = lane_read(col[k], k);
+        const bool live = d > tol;  // false for NaN as well
+        const double rkk = live ? sqrt(d) : 0.0, inv = live ? 1.0 / rkk : 0.0;
+        if (c == k)
+            col[k] = rkk;
+        else if (c > k)
+            col[k] *= inv;
+#pragma unroll
+        for (int i = k + 1; i < 16; ++i) {
+            const double aki = lane_read(col[k], i);
+            if (c >= i) col[i] -= aki * col[k];
+        }
+    }
+    double inv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) inv[r] = 0.0;
+#pragma unroll
+    for (int i = 15; i >= 0; --i) {
+        double v = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = i + 1; q < 16; ++q) v -= lane_read(col[i], q) * inv[q];
+        const double rii = lane_read(col[i], i);
+        inv[i] = (i <= c && rii > 0.0) ? v / rii : 0.0;
+    }
+    if (c < 16) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            T[r * ld + c] = (r <= c) ? col[r] : 0.0;
+            D[r * 16 + c] = inv[r];
+        }
+    }
+}
 
 __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
     __shared__ double red[16];
     const SmallPtrs P = small_ptrs(B);
     const int p = P.p;
+    const int N = (p + kNB - 1) / kNB * kNB, nb = N / kNB, LD = N + 1;
     const double *W = P.in;
     double *Rinv = P.out;
-    double *A = sm;          // p x p, upper triangle: R, then R^{-1} in place
-    double *T = sm + p * p;  // p x kNB scratch
+    double *A = sm;            // N x N at stride LD: R (upper), then R^{-1} in place
+    double *Dv = sm + N * LD;  // nb inverted diagonal tiles
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const int li = lane & 15, lq = lane >> 4;  // MFMA operand coordinates of this lane
     double tr = 0, dm = 0;
-    for (int e = tid; e < p * p; e += nt) {
-        const int i = e / p, j = e - i * p;
-        const double w = W[e];
-        A[e] = (j >= i) ? w : 0.0;
-        if (i == j) {
+    for (int e = tid; e < N * N; e += nt) {
+        const int i = e / N, j = e - i * N;
+        double w = 0.0;
+        if (i < p && j < p && j >= i) w = W[i * p + j];
+        A[i * LD + j] = w;
+        if (i == j && i < p) {
             tr += w;
             dm = fmax(dm, w);
         }
     }
     tr = wg_sum(tr, red);
     for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_down(dm, o, 64));
-    if ((tid & 63) == 0) red[tid >> 6] = dm;
+    if (lane == 0) red[wave] = dm;
     __syncthreads();
     dm = 0;
-    for (int i = 0; i < (int)(nt >> 6); ++i) dm = fmax(dm, red[i]);
+    for (int i = 0; i < nw; ++i) dm = fmax(dm, red[i]);
     __syncthreads();
     const double shift = B.shift_rel * tr / p;
     const double dead_tol = 1e-14 * (dm + shift);
-    for (int i = tid; i < p; i += nt) A[i * p + i] += shift;
+    for (int i = tid; i < p; i += nt) A[i * LD + i] += shift;
     __syncthreads();
-    const int nblk = (p + kNB - 1) / kNB;
     // ---- right-looking blocked Cholesky W = R^T R
-    for (int J = 0; J < nblk; ++J) {
-        const int j0 = J * kNB, j1 = (j0 + kNB < p) ? j0 + kNB : p;
-        if (tid < 64) {  // the diagonal block, by one wavefront (its LDS traffic stays in order)
-            for (int k = j0; k < j1; ++k) {
-                const double d = A[k * p + k];
-                const bool live = d > dead_tol;  // false for NaN as well
-                const double inv = live ? 1.0 / sqrt(d) : 0.0;
-                const int j = k + 1 + tid;
-                if (j < j1) A[k * p + j] *= inv;
-                __builtin_amdgcn_wave_barrier();
-                if (tid == 0) A[k * p + k] = live ? sqrt(d) : 0.0;
-                const int rem = j1 - k - 1;
-                for (int e = tid; e < rem * rem; e += 64) {
-                    const int i = k + 1 + e / rem, jj = k + 1 + e % rem;
-                    if (jj >= i) A[i * p + jj] -= A[k * p + i] * A[k * p + jj];
+    for (int J = 0; J < nb; ++J) {
+        const int j0 = J * kNB;
+        if (wave == 0) chol_tile(A + j0 * LD + j0, LD, Dv + 256 * J, dead_tol, lane);
+        __syncthreads();
+        // panel rows of block J: X_T = Dinv_J^T A[J, T]
+        for (int T = J + 1 + wave; T < nb; T += nw) {
+            d4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = lq + 4 * t;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Dv[256 * J + k * 16 + li], A[(j0 + k) * LD + 16 * T + li], acc,
+                                                           0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) A[(j0 + lq + 4 * v) * LD + 16 * T + li] = acc[v];
+        }
+        __syncthreads();
+        // trailing update of the tiles (I, T), J < I <= T: A[I, T] -= X_I^T X_T
+        const int m = nb - 1 - J;
+        for (int idx = wave; idx < m * (m + 1) / 2; idx += nw) {
+            int I = 0, rem = idx;
+            while (rem >= m - I) {
+                rem -= m - I;
+                ++I;
+            }
+            const int gI = J + 1 + I, gT = gI + rem;
+            d4 acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] = A[(16 * gI + lq + 4 * v) * LD + 16 * gT + li];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = lq + 4 * t;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-A[(j0 + k) * LD + 16 * gI + li], A[(j0 + k) * LD + 16 * gT + li],
+                                                           acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) A[(16 * gI + lq + 4 * v) * LD + 16 * gT + li] = acc[v];
+        }
+        __syncthreads();
+    }
+    // ---- R^{-1} in place: the inverted diagonal tiles, then block columns left to right
+    for (int e = tid; e < nb * 256; e += nt) {
+        const int K = e >> 8, r = (e >> 4) & 15, c = e & 15;
+        A[(16 * K + r) * LD + 16 * K + c] = Dv[e];
+    }
+    __syncthreads();
+    for (int J = 1; J < nb; ++J) {
+        // S_I = X[I, I..J-1] R[I..J-1, J], staged in block (J, I) of the lower triangle
+        for (int I = wave; I < J; I += nw) {
+            d4 acc = {0, 0, 0, 0};
+            for (int K = I; K < J; ++K) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int k = lq + 4 * t;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(16 * I + li) * LD + 16 * K + k],
+                                                               A[(16 * K + k) * LD + 16 * J + li], acc, 0, 0, 0);
                 }
-                __builtin_amdgcn_wave_barrier();
             }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) A[(16 * J + lq + 4 * v) * LD + 16 * I + li] = acc[v];
         }
         __syncthreads();
-        // panel rows j0..j1-1 right of the block: R_JJ^T X = A_J,c, one column per thread
-        for (int c = j1 + tid; c < p; c += nt) {
-            for (int r = j0; r < j1; ++r) {
-                double v = A[r * p + c];
-                for (int q = j0; q < r; ++q) v -= A[q * p + r] * A[q * p + c];
-                const double rr = A[r * p + r];
-                A[r * p + c] = rr > 0.0 ? v / rr : 0.0;
+        // X[I, J] = -S_I Dinv_J
+        for (int I = wave; I < J; I += nw) {
+            d4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = lq + 4 * t;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-A[(16 * J + li) * LD + 16 * I + k], Dv[256 * J + k * 16 + li],
+                                                           acc, 0, 0, 0);
             }
-        }
-        __syncthreads();
-        // trailing update of the upper triangle: A_ij -= sum over the panel rows of R_ri R_rj
-        const int rem = p - j1;
-        for (int e = tid; e < rem * rem; e += nt) {
-            const int i = j1 + e / rem, j = j1 + e % rem;
-            if (j >= i) {
-                double v = 0.0;
-                for (int r = j0; r < j1; ++r) v += A[r * p + i] * A[r * p + j];
-                A[i * p + j] -= v;
-            }
-        }
-        __syncthreads();
-    }
-    // ---- R^{-1} in place: diagonal blocks first (into T, then back), then block columns left to right
-    for (int e = tid; e < nblk * kNB; e += nt) {
-        const int J = e / kNB, cl = e - J * kNB;
-        const int j0 = J * kNB, j1 = (j0 + kNB < p) ? j0 + kNB : p;
-        const int c = j0 + cl;
-        if (c < j1) {
-            for (int i = c; i >= j0; --i) {  // column c of the block's inverse, rows i <= c
-                double v = (i == c) ? 1.0 : 0.0;
-                for (int q = i + 1; q <= c; ++q) v -= A[i * p + q] * T[q * kNB + cl];
-                const double rii = A[i * p + i];
-                T[i * kNB + cl] = rii > 0.0 ? v / rii : 0.0;
-            }
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < p * kNB; e += nt) {
-        const int i = e / kNB, cl = e - i * kNB;
-        const int J = i / kNB, c = J * kNB + cl;
-        if (c < p && c >= i) A[i * p + c] = T[i * kNB + cl];
-    }
-    __syncthreads();
-    for (int J = 1; J < nblk; ++J) {
-        const int j0 = J * kNB, j1 = (j0 + kNB < p) ? j0 + kNB : p;
-        const int jb = j1 - j0;
-        // T = X[0:j0, 0:j0] R[0:j0, j0:j1] (X upper triangular, already in place)
-        for (int e = tid; e < j0 * jb; e += nt) {
-            const int i = e / jb, cl = e - i * jb;
-            double v = 0.0;
-            for (int q = i; q < j0; ++q) v += A[i * p + q] * A[q * p + j0 + cl];
-            T[i * kNB + cl] = v;
-        }
-        __syncthreads();
-        // X[0:j0, j0:j1] = -T X[j0:j1, j0:j1]
-        for (int e = tid; e < j0 * jb; e += nt) {
-            const int i = e / jb, cl = e - i * jb;
-            double v = 0.0;
-            for (int q = 0; q <= cl; ++q) v += T[i * kNB + q] * A[(j0 + q) * p + j0 + cl];
-            A[i * p + j0 + cl] = -v;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) A[(16 * I + lq + 4 * v) * LD + 16 * J + li] = acc[v];
         }
         __syncthreads();
     }
     for (int e = tid; e < p * p; e += nt) {
         const int i = e / p, j = e - i * p;
-        Rinv[e] = (j >= i) ? A[e] : 0.0;
+        Rinv[e] = (j >= i) ? A[i * LD + j] : 0.0;
     }
 }
 
@@ -755,7 +807,8 @@ static int small_launch(std::vector<Run> &runs, F fill, void (*kern)(SmallBatch)
                         SmallBatch proto, hipStream_t s) {
     std::vector<Run *> singles;
     auto go = [&](SmallBatch &sb, int64_t pmax, int grid) -> int {
-        size_t shm = shm_per_p2 * pmax * pmax + shm_extra * pmax;
+        const int64_t pn = (pmax + kNB - 1) / kNB * kNB;  // kernels may pad p to whole 16-wide tiles
+        size_t shm = shm_per_p2 * pn * pn + shm_extra * pn;
         GTN_REQUIRE(shm + 256 <= 160 * 1024, "gtn_svd_blocks: LDS request of %zu bytes exceeds 160 KiB", shm);
         if (dry_run()) return GTN_OK;
         // LDS beyond 64 KiB must be requested explicitly (gfx950 has 160 KiB per CU)
@@ -819,7 +872,7 @@ static int orthonormalise(std::vector<Run> &runs, int passes, hipStream_t s) {
                 sb.in[i] = b.W;
                 sb.out[i] = b.R;
             },
-            k_chol_inv, sizeof(double), sizeof(double) * kNB, proto, s);
+            k_chol_inv, sizeof(double), sizeof(double) * (kNB + 1), proto, s);
         if (rc) return rc;
         std::vector<gtn_gemm_desc> g;
         for (Run &r : runs) {

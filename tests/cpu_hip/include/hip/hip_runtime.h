@@ -100,6 +100,18 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     return r;
 }
 
+// value of v in lane l (l uniform across the wavefront)
+inline int __builtin_amdgcn_readlane(int v, int l) {
+    int *buf = (int *)emu::wave_buf(0);
+    int me = emu::lane();
+    emu::sync_wave();
+    buf[me] = v;
+    emu::sync_wave();
+    int r = buf[l];
+    emu::sync_wave();
+    return r;
+}
+
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 // ---- MFMA emulation (wave64; operand/result layouts of the 16x16 instructions) -------------------
@@ -180,6 +192,12 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
+#define hipEventDisableTiming 2
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
+// device-side integer min / max of HIP
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
 
 namespace emu {
 void run_grid(dim3 grid, dim3 block, size_t shm, void (*body)(void *), void *ctx);
