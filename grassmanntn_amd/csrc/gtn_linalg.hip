@@ -33,6 +33,7 @@ static inline int64_t ext(int64_t rows, int64_t cols, int64_t ld) { return rows 
 constexpr int kPMax = 128;   // largest p x p problem held in LDS (128 KiB of FP64)
 constexpr int kMaxBatch = 8; // blocks per launch of the LDS kernels
 constexpr int kMaxSweeps = 30;  // Jacobi sweep limit (the rotation log is sized for it)
+constexpr int kJacobiTasks = 3;  // 2 x 2 blocks per thread and round: (64 * 65 / 2) / 1024 rounded up
 
 // ---------------------------------------------------------------------------------------------
 // counter-based Gaussian fill: a 64-bit mix of (seed, index) -> two uniforms -> Box-Muller
@@ -346,45 +347,69 @@ __device__ __forceinline__ void rr_pair(int t, int round, int p, int &i, int &j)
 
 // Cyclic Jacobi on a symmetric p x p matrix (p even) in LDS.  Round-robin pairing gives p/2
 // disjoint rotations per round and every pair once per sweep.  Each round computes its rotations,
-// then applies both sides at once: thread e owns the 2 x 2 block of rows {i_a, j_a} x columns
-// {i_b, j_b} for the pair of rotations (a, b), so every entry is written by one thread.  The
-// rotations (c, s) are appended to a log (in: p/2 pairs per round, pairs implied by the round);
-// k_jacobi_vectors multiplies them out into the eigenvectors afterwards.  A rotation is skipped when
-// |a_ij| <= 1e-15 sqrt(|a_ii a_jj|) or |a_ij| <= 1e-17 ||A||; the sweeps stop when a whole sweep
-// skips every rotation.  On exit lam holds the eigenvalues sorted by decreasing lambda (or
-// |lambda| when by_abs); aux[0] = rounds logged, aux[1 + c] = index of the c-th sorted eigenvalue.
+// then applies both sides at once on the upper block triangle: a thread owns the 2 x 2 block of rows
+// {i_a, j_a} x columns {i_b, j_b} for a pair of rotations a <= b and writes it and its mirror, so the
+// matrix stays exactly symmetric; blocks whose two rotations are both the identity are left alone.
+// Rounds that rotate nothing are not logged; the others append their (c, s) per pair to the log
+// (out) and their round-robin index to the index list after the log (out + kMaxSweeps (p-1) p).
+// k_jacobi_vectors multiplies the logged rounds out into the eigenvectors afterwards.  A rotation
+// is skipped when |a_ij| <= 1e-15 sqrt(|a_ii a_jj|) or |a_ij| <= 1e-17 ||A||; the sweeps stop when a
+// whole sweep skips every rotation.  On exit lam holds the eigenvalues sorted by decreasing lambda
+// (or |lambda| when by_abs); aux[0] = rounds logged, aux[1 + c] = index of the c-th sorted
+// eigenvalue.
 __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     HIP_DYNAMIC_SHARED(double, sm)
     __shared__ double red[16];
-    __shared__ int active;
+    __shared__ int active, round_mark;
     const SmallPtrs P = small_ptrs(B);
     const int p = P.p;
     const double *Cin = P.in;
     double *info = P.aux;
     double *lg = P.out;
+    double *ids = lg + (int64_t)kMaxSweeps * (p - 1) * p;
     double *lam = P.lam;
+    const int LD = p + 1;  // row stride: row and column walks of a wavefront hit distinct banks
     double *A = sm;
-    double *cs = sm + p * p;
+    double *cs = sm + p * LD;
     int *pr = (int *)(cs + p);
     int *order = pr + p;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int h = p / 2;
+    // this thread's blocks (a, b), a <= b, of the upper block triangle (the same every round)
+    const int ntask = h / 2 * (h + 1) + (h & 1) * (h + 1) / 2;
+    int my_ta[kJacobiTasks], my_tb[kJacobiTasks], my_n = 0;
+    for (int e = tid; e < ntask && my_n < kJacobiTasks; e += nt) {
+        // rows q and h-1-q of the upper triangle together hold h + 1 blocks
+        const int q = e / (h + 1), r = e - q * (h + 1);
+        if (r < h - q) {
+            my_ta[my_n] = q;
+            my_tb[my_n] = q + r;
+        } else {
+            my_ta[my_n] = h - 1 - q;
+            my_tb[my_n] = h - 1 - q + (r - (h - q));
+        }
+        ++my_n;
+    }
     double nrm = 0;
-    for (int i = tid; i < p * p; i += nt) {
-        double v = Cin[i];
-        A[i] = v;
+    for (int e = tid; e < p * p; e += nt) {
+        const int i = e / p, j = e - i * p;
+        double v = Cin[e];
+        A[i * LD + j] = v;
         nrm += v * v;
     }
-    if (tid == 0) active = 0;
+    if (tid == 0) {
+        active = 0;
+        round_mark = -1;
+    }
     nrm = wg_sum(nrm, red);
     const double abs_tol = 1e-17 * sqrt(nrm);
-    int rounds = 0;
+    int rounds = 0, nlog = 0;
     for (int sweep = 0; sweep < B.max_sweeps; ++sweep) {
         for (int round = 0; round < p - 1; ++round, ++rounds) {
             for (int t = tid; t < h; t += nt) {
                 int i, j;
                 rr_pair(t, round, p, i, j);
-                double aii = A[i * p + i], ajj = A[j * p + j], aij = A[i * p + j];
+                double aii = A[i * LD + i], ajj = A[j * LD + j], aij = A[i * LD + j];
                 double c = 1.0, s = 0.0;
                 const double fa = fabs(aij);
                 if (fa > abs_tol && fa > 1e-15 * sqrt(fabs(aii * ajj))) {
@@ -393,26 +418,45 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
                     c = 1.0 / sqrt(1.0 + tt * tt);
                     s = tt * c;
                     active = 1;
+                    round_mark = rounds;  // every writer stores the same value
                 }
                 cs[t] = c;
                 cs[h + t] = s;
                 pr[t] = i;
                 pr[h + t] = j;
-                lg[((int64_t)rounds * h + t) * 2] = c;
-                lg[((int64_t)rounds * h + t) * 2 + 1] = s;
+                lg[((int64_t)nlog * h + t) * 2] = c;
+                lg[((int64_t)nlog * h + t) * 2 + 1] = s;
             }
+            if (tid == 0) ids[nlog] = (double)round;
             __syncthreads();
-            for (int e = tid; e < h * h; e += nt) {  // A <- J^T A J, one 2 x 2 block per thread
-                const int ta = e / h, tb = e - ta * h;
-                const int i = pr[ta], j = pr[h + ta], k = pr[tb], l = pr[h + tb];
-                const double ca = cs[ta], sa = cs[h + ta], cb = cs[tb], sb = cs[h + tb];
-                const double xik = A[i * p + k], xil = A[i * p + l], xjk = A[j * p + k], xjl = A[j * p + l];
-                const double rik = ca * xik - sa * xjk, rjk = sa * xik + ca * xjk;
-                const double ril = ca * xil - sa * xjl, rjl = sa * xil + ca * xjl;
-                A[i * p + k] = cb * rik - sb * ril;
-                A[i * p + l] = sb * rik + cb * ril;
-                A[j * p + k] = cb * rjk - sb * rjl;
-                A[j * p + l] = sb * rjk + cb * rjl;
+            if (round_mark == rounds) {  // uniform: written before the barrier, rewritten only by a later round
+#pragma unroll
+                for (int x = 0; x < kJacobiTasks; ++x) {
+                    if (x >= my_n) break;
+                    const int ta = my_ta[x], tb = my_tb[x];
+                    const double sa = cs[h + ta], sb = cs[h + tb];
+                    if (sa == 0.0 && sb == 0.0) continue;
+                    const double ca = cs[ta], cb = cs[tb];
+                    const int i = pr[ta], j = pr[h + ta], k = pr[tb], l = pr[h + tb];
+                    const double xik = A[i * LD + k], xil = A[i * LD + l], xjk = A[j * LD + k], xjl = A[j * LD + l];
+                    const double rik = ca * xik - sa * xjk, rjk = sa * xik + ca * xjk;
+                    const double ril = ca * xil - sa * xjl, rjl = sa * xil + ca * xjl;
+                    const double yik = cb * rik - sb * ril, yil = sb * rik + cb * ril;
+                    const double yjk = cb * rjk - sb * rjl, yjl = sb * rjk + cb * rjl;
+                    A[i * LD + k] = yik;
+                    A[i * LD + l] = yil;
+                    A[j * LD + k] = yjk;
+                    A[j * LD + l] = yjl;
+                    if (ta != tb) {
+                        A[k * LD + i] = yik;
+                        A[l * LD + i] = yil;
+                        A[k * LD + j] = yjk;
+                        A[l * LD + j] = yjl;
+                    } else {
+                        A[j * LD + i] = yil;  // the diagonal block: keep its two off-diagonal entries equal
+                    }
+                }
+                ++nlog;
             }
             __syncthreads();
         }
@@ -425,12 +469,12 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     // sort keys: lambda (or |lambda|), with NaN ranked below everything, ties broken by index, so
     // that the ranks are a permutation of 0..p-1 whatever the values
     for (int e = tid; e < p; e += nt) {
-        double ke = A[e * p + e];
+        double ke = A[e * LD + e];
         if (B.by_abs) ke = fabs(ke);
         if (ke != ke) ke = -INFINITY;
         int rank = 0;
         for (int f = 0; f < p; ++f) {
-            double kf = A[f * p + f];
+            double kf = A[f * LD + f];
             if (B.by_abs) kf = fabs(kf);
             if (kf != kf) kf = -INFINITY;
             if (kf > ke || (kf == ke && f < e)) ++rank;
@@ -439,16 +483,16 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
     }
     __syncthreads();
     for (int i = tid; i < p; i += nt) {
-        lam[i] = A[order[i] * p + order[i]];
+        lam[i] = A[order[i] * LD + order[i]];
         info[1 + i] = (double)order[i];
     }
-    if (tid == 0) info[0] = (double)rounds;
+    if (tid == 0) info[0] = (double)nlog;
 }
 
 // Eigenvectors from the rotation log of k_jacobi: V = J_1 J_2 ... J_R applied to the identity.  Rows
 // of V evolve independently (v <- v J), so workgroup (x, y) handles rows 8x .. 8x+7 of block y: one
-// thread per (row, pair), one barrier per round.  out (p x p): column c = eigenvector of the c-th
-// sorted eigenvalue.  Launch: grid (ceil(p / 8), blocks), 8 * (p / 2) threads.
+// thread per (row, pair), one barrier per logged round.  out (p x p): column c = eigenvector of the
+// c-th sorted eigenvalue.  Launch: grid (ceil(p / 8), blocks), 8 * (p / 2) threads.
 __global__ void __launch_bounds__(512) k_jacobi_vectors(SmallBatch B) {
     __shared__ double rows[8][kPMax + 1];
     SmallPtrs P;
@@ -466,6 +510,7 @@ __global__ void __launch_bounds__(512) k_jacobi_vectors(SmallBatch B) {
     }
     const int p = P.p, h = p / 2;
     const double *lg = P.in;
+    const double *ids = lg + (int64_t)kMaxSweeps * (p - 1) * p;
     const double *info = P.aux;
     const int r0 = blockIdx.x * 8;
     const int q = threadIdx.x / h, t = threadIdx.x - q * h;  // row q of the chunk, pair t
@@ -479,7 +524,7 @@ __global__ void __launch_bounds__(512) k_jacobi_vectors(SmallBatch B) {
     for (int r = 0; r < R; ++r) {
         if (live) {
             int i, j;
-            rr_pair(t, r % (p - 1), p, i, j);
+            rr_pair(t, (int)ids[r], p, i, j);
             const double c = lg[((int64_t)r * h + t) * 2], s = lg[((int64_t)r * h + t) * 2 + 1];
             if (s != 0.0) {
                 const double vi = rows[q][i], vj = rows[q][j];
@@ -619,7 +664,7 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
     int64_t oW = take(p * p), oG = take(p * p), oR = take(p * p), oV = take(p * p), oVs = take(p * p);
     int64_t ol = take(p), oi = take(p), ok = take(p), oX = take(big * kk);
     int64_t opart = take((int64_t)32 * p * p);  // up to 32 split-K partial Gram matrices
-    int64_t olg = take((int64_t)kMaxSweeps * (p - 1) * p);  // Jacobi rotation log: (c, s) per pair
+    int64_t olg = take((int64_t)kMaxSweeps * (p - 1) * (p + 1));  // Jacobi rotation log: (c, s) per pair, round ids
     P.bytes = (off * 8 + 255) / 256 * 256;
     P.ws = ws;
     if (ws) {
@@ -908,13 +953,13 @@ static int jacobi(std::vector<Run> &runs, int by_abs, hipStream_t s) {
             sb.aux[i] = b.V;
             sb.lam[i] = b.lam;
         },
-        k_jacobi, sizeof(double), sizeof(double) + 3 * sizeof(int), proto, s);
+        k_jacobi, sizeof(double), 2 * sizeof(double) + 3 * sizeof(int), proto, s);
     if (rc) return rc;
     // eigenvectors: one launch per run (uniform) or per single block
     for (Run &r : runs) {
         Blk &f = *r.f;
         const int64_t p = f.p;
-        R8N(f.Lg, (int64_t)kMaxSweeps * (p - 1) * p, r.dws, r.n, "rotation log");
+        R8N(f.Lg, (int64_t)kMaxSweeps * (p - 1) * (p + 1), r.dws, r.n, "rotation log");
         R8N(f.V, p + 1, r.dws, r.n, "jacobi info");
         R8N(f.Vs, p * p, r.dws, r.n, "eigenvectors");
         SmallBatch sb;
