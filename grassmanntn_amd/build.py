@@ -17,6 +17,9 @@ ARCH = "gfx950"
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
          # packed FP32 instructions are kept out of all device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # matrix-core accumulators stay in the VGPRs they are used from (no per-iteration copies
+         # between accumulation and vector registers in the GEMM loops)
+         "-Xarch_device", "-mllvm=-amdgpu-mfma-vgpr-form",
          "-Wno-unused-result"]
 
 

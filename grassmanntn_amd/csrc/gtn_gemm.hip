@@ -91,8 +91,12 @@ __device__ __forceinline__ void load_panel(const T *p, int64_t sr, int64_t sk, i
             k = e / ROWS;
             r = e % ROWS;
         }
-        int gr = r0 + r, gk = k0 + k;
-        reg[s] = (gr < R && gk < K) ? ld(p, gr * sr + gk * sk) : 0.0;
+        const int gr = r0 + r, gk = k0 + k;
+        // branch-free: an out-of-range slot reads element (0, 0) of the panel's operand (valid for
+        // R, K >= 1) and is replaced by zero, so every slot's load is in flight at once
+        const bool ok = gr < R && gk < K;
+        const double v = ld(p, ok ? (int64_t)gr * sr + (int64_t)gk * sk : (int64_t)0);
+        reg[s] = ok ? v : 0.0;
     }
 }
 template <int ROWS, int BK>
@@ -141,8 +145,10 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
         for (int j = 0; j < TJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     double ra[NA], rb[NB];
     const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
-    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
-    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
+    if (K > 0) {
+        load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
+        load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
+    }
     for (int k0 = 0; k0 < K; k0 += BK) {
         __syncthreads();
 #pragma unroll
@@ -232,8 +238,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
         for (int j = 0; j < TJ; ++j) acc[i][j] = f4{0, 0, 0, 0};
     double ra[NA], rb[NB];
     const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
-    load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
-    load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
+    if (K > 0) {
+        load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
+        load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
+    }
     for (int k0 = 0; k0 < K; k0 += BK) {
         __syncthreads();
 #pragma unroll
