@@ -82,7 +82,7 @@ def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
     with gtn.precision("f64", "f64"):
         F = (gtn.logZ(T) + logn) / 2 ** (warmup + steps)
     # T' contraction: two parity-block GEMMs of (chi^2/2) x (D^2/2) x (chi^2/2)
-    key = [k for k in prof.ms if k[0] == CHI * CHI // 2 and k[2] == D * D // 2 and k[3] == 2]
+    key = [k for k in prof.ms if k[0] == CHI * CHI // 2 and k[2] == D * D // 2 and k[3] == 2 and k[5] == prec]
     tp = prof.ms[key[0]] if key else []
     return {"wall_s": wall, "gpu_ms": gpu_ms, "lnz": complex(F), "tprime_ms": tp}
 
@@ -148,6 +148,8 @@ def main():
                     "(0: the library default)")
     ap.add_argument("--svd-orth-every", type=int, default=0, help="re-orthonormalise the range-finder "
                     "basis every this many iterations (0: the library default)")
+    ap.add_argument("--svd-oversample", type=int, default=-1, help="range-finder columns beyond the kept "
+                    "rank (-1: the library default)")
     args = ap.parse_args()
     global VERBOSE
     VERBOSE = args.verbose
@@ -163,6 +165,8 @@ def main():
         gtn.settings.svd_iters = args.svd_iters
     if args.svd_orth_every:
         gtn.settings.svd_orth_every = args.svd_orth_every
+    if args.svd_oversample >= 0:
+        gtn.settings.svd_oversample = args.svd_oversample
     T0 = site_tensor(D, SEED)
     mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0, args.fp64_steps)
     fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
