@@ -595,6 +595,13 @@ def _einsum2(plan, A, B, first, ignore):
     codesA = sign_table(fa, *partA)
     codesB = sign_table(fb, *partB)
     codesO = sign_table(fo, *partO, zero_if_odd=blocked)
+    if blocked and not split and hasR and hasN:
+        # an outer product of even operands: only the even-parity row and column blocks exist (the
+        # GEMM computes that one block), so entries with an odd fused row or column are zero
+        rb = [fo.index(c) for c in RF]
+        for m in range(len(codesO)):
+            if sum((m >> b) & 1 for b in rb) & 1:
+                codesO[m] = 2
 
     def layout_for(rowsB, rowsF, hasRows, colsB, colsF, hasCols, red, gname):
         axes = [("b", c) for c in batch]

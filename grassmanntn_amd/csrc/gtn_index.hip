@@ -359,7 +359,9 @@ static int build_perm_args(const gtn_perm_desc *d, PermArgs &a) {
         GTN_REQUIRE(d->virt_dim[v] >= 1, "gtn_permute: bad virt_dim");
         GTN_REQUIRE(vmax[v] < d->virt_dim[v], "gtn_permute: virtual axis %d overflows", v);
         if (d->virt_enc[v]) {
-            GTN_REQUIRE(ilog2_exact((uint64_t)d->virt_dim[v]) >= 0, "gtn_permute: encoder on a non power-of-two axis");
+            // eps only flips the lowest bit, so it maps [0, d) onto itself for every even d (a fused
+            // index of fermionic and bosonic legs is even but need not be a power of two)
+            GTN_REQUIRE(d->virt_dim[v] % 2 == 0 || d->virt_dim[v] == 1, "gtn_permute: encoder on an odd axis");
             a.virt_enc_mask |= 1 << v;
         }
         if (d->virt_sigma[v]) a.virt_sigma_mask |= 1 << v;
@@ -394,14 +396,14 @@ static int build_perm_args(const gtn_perm_desc *d, PermArgs &a) {
         GTN_REQUIRE(fmax[b] < d->in_dim[b], "gtn_permute: input axis %d overflows (%lld >= %lld)", b, (long long)fmax[b],
                     (long long)d->in_dim[b]);
         if (d->in_enc[b]) {
-            GTN_REQUIRE(ilog2_exact((uint64_t)d->in_dim[b]) >= 0, "gtn_permute: encoder on a non power-of-two input axis");
+            GTN_REQUIRE(d->in_dim[b] % 2 == 0 || d->in_dim[b] == 1, "gtn_permute: encoder on an odd input axis");
             a.in_enc_mask |= 1 << b;
         }
         if (d->in_sigma[b]) a.in_sigma_mask |= 1 << b;
         GTN_REQUIRE(d->in_stride[b] >= 0 && d->in_stride_lo[b] >= 0, "gtn_permute: negative stride");
         a.in_stride[b] = d->in_stride[b];
         a.in_stride_lo[b] = d->in_stride_lo[b];
-        // the largest encoded index is in_dim - 1 (eps maps [0, 2^n) onto itself)
+        // the largest encoded index is in_dim - 1 (eps maps [0, d) onto itself for even d)
         int64_t emax = d->in_dim[b] - 1;
         if (d->in_split[b]) {
             a.in_split_mask |= 1 << b;

@@ -1027,6 +1027,95 @@ def atrg2dx(T1, T2, dcut=64, intermediate_dcut=None, error_test=False, mode="sec
     return (T,) + tuple(res[1:])
 
 
+def hotrg3dz(T1, T2, dcut=64, intermediate_dcut=None, error_test=False, mode="sector", block_fn=None):
+    """One 3-D HOTRG step along z (gauge2d.py:1891-2145): each site tensor (i1 i2 i3 i4 m n) is split
+    by two SVDs into X (x legs), P, Q and Y (y legs); the isometries Ux, Uy are the eig of the
+    squared four-leg blocks (the one of the two candidates with fewer kept values, as the
+    reference's S.shape[0] comparison picks), and T' = (X' Y') with legs (t1 t2 t3 t4 m n).  The
+    sequence of contractions and transpositions is the reference's, so that every sign matches."""
+    T1o, T2o = T1, T2
+    if intermediate_dcut is None:
+        intermediate_dcut = dcut
+    e = lambda sub, *ops: einsum(sub, *ops, mode=mode)
+
+    def split(T):
+        T = e("i1 i2 i3 i4 mn-> i1 i3 mn i2 i4", T)
+        X, S, Y = svd(T, "(i1 i3 m)(n i2 i4)", intermediate_dcut, block_fn=block_fn)
+        r = sqrt(S)
+        X = e("i1 i3 m a,ab->i1 i3 b m", X, r)
+        Y = e("ab,b n i2 i4->n a i2 i4", r, Y)
+        X, S, P = svd(X, "(i1 i3)(a m)", intermediate_dcut, block_fn=block_fn)
+        X = e("i1 i3 x, xy -> i1 i3 y", X, S)
+        Q, S, Y = svd(Y, "(na)(i2 i4)", intermediate_dcut, block_fn=block_fn)
+        Y = e("xy, y i2 i4 -> x i2 i4", S, Y)
+        return X, P, Q, Y
+
+    X1, P1, Q1, Y1 = split(T1)
+    X2, P2, Q2, Y2 = split(T2)
+    XX = e("i1 i3 a, j1 j3 b -> i1 i3 ab j1 j3", X1, X2)
+    XX = e("i1 i3 ab j1 j3 -> i1 i3 ab j3 j1 ", XX)
+    XX = e("i1 i3 ab j3 j1 -> i1 i3 j3 ab j1 ", XX)
+    XX = e("i1 i3 j3 ab j1 -> i3 j3 i1 ab j1 ", XX)
+    XX = e("i3 j3 i1 ab j1 -> i3 j3 ab i1 j1 ", XX)
+    c1 = hconjugate(XX, "(i3 j3 ab)(i1 j1)")
+    M1 = e(" I1 J1 i3 j3 ab, i3 j3 ab i1 j1 -> I1 J1 i1 j1", c1, XX)
+    U1, S1, _ = eig(M1, "(I J)(i j)", dcut, block_fn=block_fn)
+    c3 = hconjugate(XX, "(i3 j3)(ab i1 j1)")
+    M3 = e(" I3 J3 ab i1 j1, ab i1 j1 i3 j3  -> I3 J3 i3 j3", XX, c3)
+    U3, S3, _ = eig(M3, "(I J)(i j)", dcut, block_fn=block_fn)
+    Ux = U1 if S1.shape[0] < S3.shape[0] else U3
+    cUx = hconjugate(Ux, "ij|a")
+    XX = e("i3 j3 ab i1 j1 -> j3 i3 ab i1 j1", XX)
+    XX = e("j3 i3 ab i1 j1 -> j3 i3 ab j1 i1", XX)
+    Xp = e("s i3 j3,j3 i3 kl j1 i1 -> s kl j1 i1", cUx, XX)
+    Xp = e("s kl j1 i1, i1 j1 t -> s kl t", Xp, Ux)
+    Xp = e("s kl t -> t s kl", Xp)
+    Xp = e("t s kl , kam -> t s al m", Xp, P1)
+    Xp = e("t s al m , lbm -> t s ab m", Xp, P2)
+    YY = e("b j2 j4, a i2 i4 -> j2 j4 ba i2 i4 ", Y2, Y1)
+    YY = e("j2 j4 ba i2 i4 -> j2 j4 i4 ba i2 ", YY)
+    YY = e("j2 j4 i4 ba i2 -> j4 i4 j2 ba i2 ", YY)
+    YY = e("j4 i4 j2 ba i2 -> j4 i4 ba i2 j2 ", YY)
+    YY = e("j4 i4 ba i2 j2 -> i4 j4 ba i2 j2 ", YY)
+    c2 = hconjugate(YY, "(i4 j4 ba)(i2 j2)")
+    M2 = e(" I2 J2 i4 j4 ba, i4 j4 ba i2 j2 -> I2 J2 i2 j2", c2, YY)
+    U2, S2, _ = eig(M2, "(I J)(i j)", dcut, block_fn=block_fn)
+    c4 = hconjugate(YY, "(i4 j4)(ba i2 j2)")
+    M4 = e(" I4 J4 ba i2 j2, ba i2 j2 i4 j4  -> I4 J4 i4 j4", YY, c4)
+    U4, S4, _ = eig(M4, "(I J)(i j)", dcut, block_fn=block_fn)
+    Uy = U2 if S2.shape[0] < S4.shape[0] else U4
+    cUy = hconjugate(Uy, "ij|a")
+    YY = e("i4 j4 b a i2 j2 -> j4 i4 b a i2 j2", YY)
+    YY = e("j4 i4 b a i2 j2 -> j4 i4 b a j2 i2", YY)
+    Yp = e("s i4 j4,j4 i4 lk j2 i2 -> s lk j2 i2", cUy, YY)
+    Yp = e("s lk j2 i2, i2 j2 t -> s lk t", Yp, Uy)
+    Yp = e("s lk t -> lk t s", Yp)
+    Yp = e("nak, lk t s -> n la t s", Q1, Yp)
+    Yp = e("nbl, n la t s -> n ba t s", Q2, Yp)
+    T = e(" t1 t3 kl m, n lk t2 t4 -> t1 t2 t3 t4 mn ", Xp, Yp)
+    err = None
+    if error_test:
+        Z1 = e("IJIJmn,KLKLmn->mn", T1o, T2o)
+        Z2 = e("IJIJmn->mn", T)
+        err = (Z1 - Z2).norm / Z1.norm
+    nrm = T.norm
+    T = T / nrm
+    return (T, nrm, err) if error_test else (T, nrm)
+
+
+def logZhotrg3dz(T1, T2, boundary_conditions="periodic", mode="sector"):
+    """ln of einsum('IJIJmn,KLKLmn') (gauge2d.py:1617-1641); anti-periodic signs (-1)^popcount of
+    the stored index along leg J of both tensors."""
+    T1, T2 = T1.copy(), T2.copy()
+    if boundary_conditions == "anti-periodic":
+        d = T1.shape[1]
+        s = (-1) ** gparity(np.arange(d))
+        T1.data = np.einsum("IJKLmn,J->IJKLmn", T1.data, s)
+        T2.data = np.einsum("IJKLmn,J->IJKLmn", T2.data, s)
+    Z = einsum("IJIJmn,KLKLmn", T1, T2, mode=mode)
+    return np.log(Z + 0j) if np.isreal(Z) and Z < 0 else np.log(Z)
+
+
 def zcap(T, mode="sector"):
     """Trace the two bosonic flavour legs with an all-ones capper (gauge2d.py:1591-1597)."""
     n = T.shape[4]

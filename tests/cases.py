@@ -1,5 +1,6 @@
 """Contraction patterns shared by the oracle and GPU parity tests: the TRG / ATRG / zcap einsums of
-gauge2d.py plus general shapes (mixed statistics, bosonic legs, traces, three operands)."""
+gauge2d.py plus general shapes (mixed statistics, bosonic legs, traces, three operands, outer
+products)."""
 
 EINSUMS = [
     ("ilk,jl->ijk", [(1, 1, -1), (-1, -1)], [(4, 8, 4), (8, 8)]),
@@ -17,4 +18,8 @@ EINSUMS = [
     ("ab,bc,cd->ad", [(1, 1), (-1, 1), (-1, 1)], [(4, 4), (4, 4), (4, 4)]),
     ("iIaj,jIbk->akbi", [(1, 0, -1, 1), (-1, 0, 1, 1)], [(4, 3, 8, 4), (4, 3, 2, 8)]),
     ("IJKLij,ij->IJKL", [(1, 1, -1, -1, 0, 0), (0, 0)], [(4, 4, 4, 4, 2, 2), (2, 2)]),
+    # outer products (no contracted index), as in hotrg3dz's XX / YY (gauge2d.py:1962, :2033)
+    ("ia,jb->iajb", [(1, 1), (1, -1)], [(4, 8), (8, 4)]),
+    ("i1 i3 a, j1 j3 b -> i1 i3 ab j1 j3", [(1, -1, 1), (1, -1, 1)], [(4, 4, 8), (4, 4, 2)]),
+    ("ija,k->iakj", [(1, -1, 0), (1,)], [(4, 8, 3), (2,)]),
 ]

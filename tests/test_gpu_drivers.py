@@ -96,3 +96,25 @@ def test_trg_c0_precision_schedule(prec):
     F = _lnz_c0(lambda i: ("f64", "f64") if i < 3 else (prec, "f32"))
     print("C0 |dlnZ| schedule 3xf64 + %s: %.3e" % (prec, abs(F - F0)))
     assert abs(F - F0) < 1e-7, (prec, F, F0)
+
+
+@pytest.mark.parametrize("dims,dcut", [((4, 4, 4, 4, 2, 2), 64), ((4, 4, 4, 4, 2, 2), 4), ((4, 4, 4, 4, 3, 3), 8)])
+def test_hotrg3dz_matches_oracle(dims, dcut):
+    """3-D HOTRG step (gauge2d.py:1891-2145) with bosonic z legs: norm, error_test value and the
+    periodic / anti-periodic logZhotrg3dz of the result against the oracle; at dcut 64 nothing is
+    truncated and the error is the Z(T1, T2) = Z(T') identity."""
+    rng = np.random.default_rng(sum(dims) + dcut)
+    st = (1, 1, -1, -1, 0, 0)
+    T1 = random_even(rng, dims, st, "parity-preserving")
+    T2 = random_even(rng, dims, st, "parity-preserving")
+    ref, nref, eref = O.hotrg3dz(T1, T2, dcut, error_test=True)
+    out, nrm, err = gtn.hotrg3dz(to_dev(T1), to_dev(T2), dcut, error_test=True)
+    assert tuple(out.shape) == tuple(ref.shape) and out.statistics == ref.statistics
+    assert abs(nrm - nref) <= 1e-10 * nref, (nrm, nref)
+    assert abs(err - eref) <= 1e-10, (err, eref)
+    if dcut == 64:
+        assert err < 1e-12
+    for bc in ("periodic", "anti-periodic"):
+        z_ref = O.logZhotrg3dz(ref, ref, bc)
+        z = gtn.logZhotrg3dz(out, out, bc)
+        assert abs(z - z_ref) <= 1e-10 * max(1.0, abs(z_ref)), (bc, z, z_ref)

@@ -91,3 +91,30 @@ def test_site_tensor_generator():
     assert T.shape == (16,) * 4 and T.encoder == "parity-preserving" and T.statistics == (1, 1, -1, -1)
     assert abs(np.abs(T.data).max() - 1.0) < 1e-15
     assert O.is_grassmann_even(T)
+
+
+def _site6(rng, dims):
+    """A Grassmann-even site tensor (i1 i2 i3 i4 m n) with fermionic x/y legs and bosonic z legs."""
+    return random_even(rng, dims, (1, 1, -1, -1, 0, 0), "parity-preserving")
+
+
+def test_hotrg3dz_error_identity_at_full_bond(rng):
+    """At bonds that truncate nothing the coarse-grained tensor keeps the traced partition function:
+    Z(T1, T2) = Z(T') per z configuration (the reference's error_test, gauge2d.py:2133-2136).  Any
+    wrong sign among the step's 40 contractions and transpositions breaks this identity."""
+    T1, T2 = _site6(rng, (4, 4, 4, 4, 2, 2)), _site6(rng, (4, 4, 4, 4, 2, 2))
+    T, nrm, err = O.hotrg3dz(T1, T2, dcut=64, error_test=True)
+    assert err < 1e-12 and nrm > 0
+    assert T.shape == (16, 16, 16, 16, 2, 2) and T.statistics == (1, 1, -1, -1, 0, 0)
+    for bc in ("periodic", "anti-periodic"):
+        assert np.isfinite(O.logZhotrg3dz(T, T, bc))
+
+
+def test_hotrg3dz_literal_equals_sector(rng):
+    """The literal (per-element sign tensor) einsum and the per-parity-sector einsum give the same
+    truncated HOTRG step."""
+    T1, T2 = _site6(rng, (4, 4, 4, 4, 2, 2)), _site6(rng, (4, 4, 4, 4, 2, 2))
+    a, na = O.hotrg3dz(T1, T2, dcut=6, mode="literal")
+    b, nb = O.hotrg3dz(T1, T2, dcut=6, mode="sector")
+    assert abs(na - nb) < 1e-12 * na
+    assert abs(O.logZhotrg3dz(a, a) - O.logZhotrg3dz(b, b)) < 1e-10
