@@ -416,6 +416,70 @@ def _out_encoder_pp(first, stat):
     return first.encoder == "parity-preserving" and stat in FERMI
 
 
+class _PermutedView(dense):
+    """A pure leg permutation of a dense tensor (an einsum such as 'ijkl->jkli') whose gather is
+    deferred: `join_legs` reads the source directly with the permutation's sign table folded into its
+    own gather; any other use materialises the permuted tensor on first access of `.data`."""
+
+    def __init__(self, src, order, codes, fbits, stats):
+        self._src, self._order, self._codes, self._fbits = src, tuple(order), list(codes), list(fbits)
+        self._data = None
+        self.statistics = tuple(stats)
+        self.encoder, self.format = src.encoder, src.format
+        self._even = src._even
+
+    @property
+    def data(self):
+        if self._data is None:
+            self._data = _gather_permuted(self._src, self._order, self._codes, self._fbits)
+        return self._data
+
+    @data.setter
+    def data(self, value):
+        self._data = value
+
+    shape = property(lambda self: tuple(self._src.shape[a] for a in self._order) if self._data is None
+                     else tuple(self._data.shape))
+    ndim = property(lambda self: len(self._order))
+
+
+def _gather_permuted(src, order, codes, fbits):
+    """out[x] = code(parities of x) * src[x permuted]: out axis a is source axis order[a]; bit k of the
+    table index is the parity of out axis fbits[k]."""
+    P = Perm()
+    st = strides(src.shape)
+    ins = [P.add_in(src.shape[b], st[b], enc=_is_pp(src, b), sigma=(src.format == "matrix" and src.statistics[b] == -1))
+           for b in range(src.ndim)]
+    for a, b in enumerate(order):
+        pp = _is_pp(src, b)
+        sig = src.format == "matrix" and src.statistics[b] == -1
+        v = P.add_virt(src.shape[b], enc=pp, sigma=sig)
+        P.add_comp(v, src.shape[b], [(ins[b], 1)], bit=fbits.index(a) if a in fbits else -1)
+        P.add_out(src.shape[b], [(v, 1)])
+    P.set_table(codes)
+    return P.run(src.data)
+
+
+def permuted_view(T, subscripts):
+    """einsum(subscripts, T) for a pure permutation of T's legs (each index once on both sides), with
+    the gather deferred (see _PermutedView); other subscripts run einsum."""
+    sub = denumerate(subscripts.replace(" ", ""))
+    if "->" not in sub or not isinstance(T, dense) or isinstance(T, _PermutedView) or T.data.is_complex():
+        return einsum(subscripts, T)
+    s, out = sub.split("->")
+    if len(set(s)) != len(s) or sorted(s) != sorted(out) or len(s) != T.ndim:
+        return einsum(subscripts, T)
+    plan = contraction(s + "->" + out, [(T.statistics, T.shape)])
+    if plan.pairs:
+        return einsum(subscripts, T)
+    fa = [c for c in dict.fromkeys(s) if _fermionic(plan, c)]
+    lin, quad = plan.quadratic_form()
+    codes = sign_table(fa, lin, quad)
+    order = [s.index(c) for c in out]
+    fbits = [out.index(c) for c in fa]
+    return _PermutedView(T, order, codes, fbits, plan.out_stats)
+
+
 def _einsum1(plan, A, first, ignore):
     s = plan.inputs[0]
     out = plan.output
@@ -774,11 +838,23 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
     sgn_pos = _grouping_sign_axes(srt, inter)
     info = _intermediate(srt, inter)
     P = Perm()
-    st = strides(T.shape)
+    view = (isinstance(T, _PermutedView) and T._data is None and T.format == "standard"
+            and len(set(sgn_pos) | set(T._fbits)) <= 10)
     std = T.format == "matrix"  # input must first go to standard: sigma on its -1 legs
-    ins = [P.add_in(T.shape[a], st[a], enc=_is_pp(T, a), sigma=(std and T.statistics[a] == -1)) for a in range(T.ndim)]
-    nb = len(sgn_pos)
-    bitpos = {p: k for k, p in enumerate(sorted(sgn_pos))}
+    if view:
+        # read the permuted view's source directly: T axis a is source axis order[a]; the view's
+        # sign table joins the grouping sign in this gather's table
+        st_src = strides(T._src.shape)
+        ins = [P.add_in(T.shape[a], st_src[T._order[a]], enc=_is_pp(T, a)) for a in range(T.ndim)]
+        src_data = T._src.data
+    else:
+        st = strides(T.shape)
+        ins = [P.add_in(T.shape[a], st[a], enc=_is_pp(T, a), sigma=(std and T.statistics[a] == -1))
+               for a in range(T.ndim)]
+        src_data = T.data
+    bit_axes = sorted(set(sgn_pos) | (set(T._fbits) if view else set()))
+    nb = len(bit_axes)
+    bitpos = {p: k for k, p in enumerate(bit_axes)}
     fin_shape, fin_stats = [], []
     for g, (chars, stg, dm, pos) in enumerate(srt):
         bd, fd, bn, fn, final = info[g]
@@ -811,8 +887,15 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
         fin_shape.append(bd * fd)
         fin_stats.append(final)
     if nb:
-        P.set_table([bin(m).count("1") & 1 for m in range(1 << nb)])
-    out = T._like(P.run(T.data), statistics=fin_stats, encoder="parity-preserving", format=make_format,
+        table = []
+        for m in range(1 << nb):
+            code = sum((m >> bitpos[p]) & 1 for p in sgn_pos) & 1
+            if view:
+                t = T._codes[sum(((m >> bitpos[a]) & 1) << k for k, a in enumerate(T._fbits))]
+                code = 2 if t == 2 else code ^ t
+            table.append(code)
+        P.set_table(table)
+    out = T._like(P.run(src_data), statistics=fin_stats, encoder="parity-preserving", format=make_format,
                   even=T._even)
     out.data = out.data.reshape(fin_shape)
     return out
@@ -924,7 +1007,7 @@ def _block_assemble(outs, kk, m, n, dt, kept=None):
 
 def _interleave(stack, rows, cols, row_stride, par_stride):
     """full[r][c] = stack[r % 2][r // 2][c // 2] when r and c have equal parity, else 0 (both dims
-    powers of two): the interleaving of get_full_matrix (__init__.py:4017-4028)."""
+    even): the interleaving of get_full_matrix (__init__.py:4017-4028)."""
     P = Perm()
     ir = P.add_in(rows, row_stride, enc=True, split=True, stride_lo=par_stride)
     ic = P.add_in(cols, 1, enc=True, split=True, stride_lo=0)
@@ -1070,7 +1153,8 @@ def _decompose_prepare(T, string, cutoff, mode, rule):
     name = "svd" if mode == 0 else "eig"
     if not isinstance(T, dense):
         fail("Error[%s]: Object type must only be dense or sparse!" % name)
-    if T.data.is_complex() or T.data.dtype != np.float64:
+    Td = T._src.data if isinstance(T, _PermutedView) and T._data is None else T.data
+    if Td.is_complex() or Td.dtype != np.float64:
         raise NotImplementedError("%s of non-FP64-real tensors is not provided by this build" % name)
     for st in T.statistics:
         if st not in ALLOWED_STAT:

@@ -456,14 +456,16 @@ static int pick_tile(const gtn_gemm_desc *groups, int n) {
     return t128 >= 256 ? 128 : 64;
 }
 
-// GTN_GEMM_SPLIT=0 in the environment turns the split reduction off (tuning and tests).
-static bool split_enabled() {
-    static int on = -1;
-    if (on < 0) {
+// GTN_GEMM_SPLIT=0 in the environment turns the split reduction off; GTN_GEMM_SPLIT=n > 1 sets the
+// workgroup count a split launch aims for (default 1024) (tuning and tests).
+static int split_target() {
+    static int target = -1;
+    if (target < 0) {
         const char *e = getenv("GTN_GEMM_SPLIT");
-        on = (e && atoi(e) == 0) ? 0 : 1;
+        const int v = e ? atoi(e) : 1024;
+        target = v == 0 ? 0 : (v > 1 ? v : 1024);
     }
-    return on == 1;
+    return target;
 }
 
 }  // namespace gtn
@@ -543,14 +545,16 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
         }
         // Too few tiles to give every compute unit two workgroups: cut long reductions into chunks
         // (partials to the workspace, summed in a fixed order afterwards) until the launch has about
-        // four workgroups per compute unit.  A chunk keeps at least 256 of the reduction.
+        // four workgroups per compute unit (the split target).  A chunk keeps at least 256 of the
+        // reduction.
         const int es = dtype == GTN_F64 ? 8 : 4;
         std::vector<GemmGroup> orig;
         std::unique_lock<std::mutex> lk(g_ws_mu, std::defer_lock);
         SplitWs *ws = nullptr;
-        if (tiles < 512 && split_enabled()) {
+        const int target = split_target();
+        if (tiles < 512 && target > 0 && tiles * 2 <= target) {
             int S = 1;
-            while (tiles * S < 1024 && S < 16) S *= 2;
+            while (tiles * S < target && S < 16) S *= 2;
             int64_t need = 0;
             int splitting = 0;
             for (int i = 0; i < used; ++i) {

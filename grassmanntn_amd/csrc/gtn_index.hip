@@ -832,6 +832,16 @@ static int tab_tables(const PermArgs &a, const TabBuild &tb, hipStream_t s, TabA
     return GTN_OK;
 }
 
+// GTN_PERM_TILED=0 in the environment turns the LDS-tiled gather off (tuning and tests).
+static bool tiles_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("GTN_PERM_TILED");
+        on = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    return on == 1;
+}
+
 int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream) {
     GTN_REQUIRE(d && src && dst, "gtn_permute: null argument");
     PermArgs a;
@@ -854,7 +864,7 @@ int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream
         if (rc) return rc;
     }
     TileArgs t;
-    if (d->dtype != GTN_C128 && plan_tiles(d, t)) {
+    if (d->dtype != GTN_C128 && tiles_enabled() && plan_tiles(d, t)) {
         const int64_t nblk = (int64_t)t.tiles0 * t.tiles1 * t.rest_total;
         dim3 gt((unsigned)(nblk < (1 << 20) ? nblk : (1 << 20))), bt(256);
         if (d->dtype == GTN_F64) {

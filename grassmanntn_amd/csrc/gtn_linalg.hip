@@ -490,11 +490,13 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
 }
 
 // Eigenvectors from the rotation log of k_jacobi: V = J_1 J_2 ... J_R applied to the identity.  Rows
-// of V evolve independently (v <- v J), so workgroup (x, y) handles rows 8x .. 8x+7 of block y: one
-// thread per (row, pair), one barrier per logged round.  out (p x p): column c = eigenvector of the
-// c-th sorted eigenvalue.  Launch: grid (ceil(p / 8), blocks), 8 * (p / 2) threads.
-__global__ void __launch_bounds__(512) k_jacobi_vectors(SmallBatch B) {
-    __shared__ double rows[8][kPMax + 1];
+// of V evolve independently (v <- v J), so one wavefront owns one row of one block: lane t applies
+// pair t of every logged round to the row held in LDS, with only wavefront-level ordering between
+// rounds, and the next round's (c, s) and round index are fetched while the current one is applied.
+// out (p x p): column c = eigenvector of the c-th sorted eigenvalue.  Launch: grid (p, blocks), 64
+// threads (p <= 128).
+__global__ void __launch_bounds__(64) k_jacobi_vectors(SmallBatch B) {
+    __shared__ double row[kPMax];
     SmallPtrs P;
     const int64_t bi = blockIdx.y;
     if (B.uniform) {
@@ -509,35 +511,44 @@ __global__ void __launch_bounds__(512) k_jacobi_vectors(SmallBatch B) {
         P.aux = B.aux[bi];
     }
     const int p = P.p, h = p / 2;
+    const int r0 = blockIdx.x;
+    if (r0 >= p) return;
     const double *lg = P.in;
     const double *ids = lg + (int64_t)kMaxSweeps * (p - 1) * p;
     const double *info = P.aux;
-    const int r0 = blockIdx.x * 8;
-    const int q = threadIdx.x / h, t = threadIdx.x - q * h;  // row q of the chunk, pair t
-    const bool live = threadIdx.x < 8 * h && r0 + q < p;
-    for (int e = threadIdx.x; e < 8 * p; e += blockDim.x) {
-        const int rq = e / p, c = e - rq * p;
-        rows[rq][c] = (r0 + rq == c) ? 1.0 : 0.0;
-    }
-    __syncthreads();
+    const int t = threadIdx.x;
+    const bool live = t < h;
+    for (int c = t; c < p; c += 64) row[c] = (c == r0) ? 1.0 : 0.0;
+    __builtin_amdgcn_wave_barrier();
     const int R = (int)info[0];
+    double c = 1.0, s = 0.0;
+    int id = 0;
+    if (R > 0 && live) {
+        c = lg[(int64_t)t * 2];
+        s = lg[(int64_t)t * 2 + 1];
+        id = (int)ids[0];
+    }
     for (int r = 0; r < R; ++r) {
-        if (live) {
-            int i, j;
-            rr_pair(t, (int)ids[r], p, i, j);
-            const double c = lg[((int64_t)r * h + t) * 2], s = lg[((int64_t)r * h + t) * 2 + 1];
-            if (s != 0.0) {
-                const double vi = rows[q][i], vj = rows[q][j];
-                rows[q][i] = c * vi - s * vj;
-                rows[q][j] = s * vi + c * vj;
-            }
+        double cn = 1.0, sn = 0.0;
+        int idn = 0;
+        if (r + 1 < R && live) {
+            cn = lg[((int64_t)(r + 1) * h + t) * 2];
+            sn = lg[((int64_t)(r + 1) * h + t) * 2 + 1];
+            idn = (int)ids[r + 1];
         }
-        __syncthreads();
+        if (live && s != 0.0) {
+            int i, j;
+            rr_pair(t, id, p, i, j);
+            const double vi = row[i], vj = row[j];
+            row[i] = c * vi - s * vj;
+            row[j] = s * vi + c * vj;
+        }
+        __builtin_amdgcn_wave_barrier();
+        c = cn;
+        s = sn;
+        id = idn;
     }
-    for (int e = threadIdx.x; e < 8 * p; e += blockDim.x) {
-        const int rq = e / p, c = e - rq * p;
-        if (r0 + rq < p) P.out[(int64_t)(r0 + rq) * p + c] = rows[rq][(int)info[1 + c]];
-    }
+    for (int e = t; e < p; e += 64) P.out[(int64_t)r0 * p + e] = row[(int)info[1 + e]];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -971,7 +982,7 @@ static int jacobi(std::vector<Run> &runs, int by_abs, hipStream_t s) {
         sb.out[0] = f.Vs;
         sb.aux[0] = f.V;
         sb.d_in = sb.d_out = sb.d_aux = r.dws;
-        LAUNCH(k_jacobi_vectors, dim3((unsigned)((p + 7) / 8), r.n), dim3((unsigned)(8 * (p / 2))), 0, s, sb);
+        LAUNCH(k_jacobi_vectors, dim3((unsigned)p, r.n), dim3(64), 0, s, sb);
     }
     return GTN_OK;
 }

@@ -52,8 +52,13 @@ def trg(T, dcut=64, iternum=None, error_test=False):
         fail("Error[trg]: The shape must be of the form (m,n,m,n)!")
     if list(T.statistics) != [1, 1, -1, -1]:
         fail("Error[trg]: The statistics must be (1,1,-1,-1)!")
-    T1 = gtn.einsum("ijkl->jkli", T)
-    T2 = gtn.einsum("ijkl->klij", T)
+    if isinstance(T, gtn.dense):
+        # the two permutations are folded into the decompositions' join gathers
+        T1 = gtn.permuted_view(T, "ijkl->jkli")
+        T2 = gtn.permuted_view(T, "ijkl->klij")
+    else:
+        T1 = gtn.einsum("ijkl->jkli", T)
+        T2 = gtn.einsum("ijkl->klij", T)
     if isinstance(T1, gtn.dense):
         # both decompositions' parity blocks in one batched device call
         (U1, S1, V1), (U2, S2, V2) = gtn.svd_many([(T1, "ab|cd", dcut), (T2, "ab|cd", dcut)])
