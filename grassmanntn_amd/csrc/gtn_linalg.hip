@@ -338,9 +338,14 @@ __global__ void __launch_bounds__(1024) k_chol_inv(SmallBatch B) {
 }
 
 // round-robin pairing of round `round` (p even): pair t is (a, b) with a != b, every index once
+// (t < p/2 and 0 <= round < p - 1 keep both sums below 2 (p - 1): one conditional subtraction is the
+// modulo)
 __device__ __forceinline__ void rr_pair(int t, int round, int p, int &i, int &j) {
-    const int a = (t == 0) ? 0 : ((t - 1 + round) % (p - 1)) + 1;
-    const int b = ((p - 2 - t + round) % (p - 1)) + 1;
+    int a0 = t - 1 + round, b0 = p - 2 - t + round;
+    if (a0 >= p - 1) a0 -= p - 1;
+    if (b0 >= p - 1) b0 -= p - 1;
+    const int a = (t == 0) ? 0 : a0 + 1;
+    const int b = b0 + 1;
     i = a < b ? a : b;
     j = a < b ? b : a;
 }
@@ -492,7 +497,7 @@ __global__ void __launch_bounds__(1024) k_jacobi(SmallBatch B) {
 // Eigenvectors from the rotation log of k_jacobi: V = J_1 J_2 ... J_R applied to the identity.  Rows
 // of V evolve independently (v <- v J), so one wavefront owns one row of one block: lane t applies
 // pair t of every logged round to the row held in LDS, with only wavefront-level ordering between
-// rounds, and the next round's (c, s) and round index are fetched while the current one is applied.
+// rounds.
 // out (p x p): column c = eigenvector of the c-th sorted eigenvalue.  Launch: grid (p, blocks), 64
 // threads (p <= 128).
 __global__ void __launch_bounds__(64) k_jacobi_vectors(SmallBatch B) {
@@ -521,32 +526,43 @@ __global__ void __launch_bounds__(64) k_jacobi_vectors(SmallBatch B) {
     for (int c = t; c < p; c += 64) row[c] = (c == r0) ? 1.0 : 0.0;
     __builtin_amdgcn_wave_barrier();
     const int R = (int)info[0];
-    double c = 1.0, s = 0.0;
-    int id = 0;
-    if (R > 0 && live) {
-        c = lg[(int64_t)t * 2];
-        s = lg[(int64_t)t * 2 + 1];
-        id = (int)ids[0];
-    }
-    for (int r = 0; r < R; ++r) {
-        double cn = 1.0, sn = 0.0;
-        int idn = 0;
-        if (r + 1 < R && live) {
-            cn = lg[((int64_t)(r + 1) * h + t) * 2];
-            sn = lg[((int64_t)(r + 1) * h + t) * 2 + 1];
-            idn = (int)ids[r + 1];
+    // rounds are applied in chunks of kVecChunk whose (c, s, round index) are fetched one chunk
+    // ahead, so that a chunk's loads are in flight while the previous chunk is applied
+    constexpr int kVecChunk = 8;
+    double cc[kVecChunk], ss[kVecChunk];
+    int ii[kVecChunk];
+    auto fetch = [&](int base, double (&c)[kVecChunk], double (&s)[kVecChunk], int (&id)[kVecChunk]) {
+#pragma unroll
+        for (int q = 0; q < kVecChunk; ++q) {
+            const int r = base + q;
+            const bool ok = live && r < R;
+            c[q] = ok ? lg[((int64_t)r * h + t) * 2] : 1.0;
+            s[q] = ok ? lg[((int64_t)r * h + t) * 2 + 1] : 0.0;
+            id[q] = ok ? (int)ids[r] : 0;
         }
-        if (live && s != 0.0) {
-            int i, j;
-            rr_pair(t, id, p, i, j);
-            const double vi = row[i], vj = row[j];
-            row[i] = c * vi - s * vj;
-            row[j] = s * vi + c * vj;
+    };
+    fetch(0, cc, ss, ii);
+    for (int base = 0; base < R; base += kVecChunk) {
+        double cn[kVecChunk], sn[kVecChunk];
+        int in[kVecChunk];
+        fetch(base + kVecChunk, cn, sn, in);
+#pragma unroll
+        for (int q = 0; q < kVecChunk; ++q) {
+            if (live && ss[q] != 0.0) {
+                int i, j;
+                rr_pair(t, ii[q], p, i, j);
+                const double vi = row[i], vj = row[j];
+                row[i] = cc[q] * vi - ss[q] * vj;
+                row[j] = ss[q] * vi + cc[q] * vj;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
-        c = cn;
-        s = sn;
-        id = idn;
+#pragma unroll
+        for (int q = 0; q < kVecChunk; ++q) {
+            cc[q] = cn[q];
+            ss[q] = sn[q];
+            ii[q] = in[q];
+        }
     }
     for (int e = t; e < p; e += 64) P.out[(int64_t)r0 * p + e] = row[(int)info[1 + e]];
 }

@@ -112,6 +112,21 @@ inline int __builtin_amdgcn_readlane(int v, int l) {
     return r;
 }
 
+// value of v in lane src (src may differ per lane; lanes outside [0, width) read their own value)
+template <typename T>
+inline T __shfl(T v, int src, int width = 64) {
+    static_assert(sizeof(T) <= 8, "shuffle of a value wider than 8 bytes");
+    T *buf = (T *)emu::wave_buf(0);
+    int l = emu::lane();
+    emu::sync_wave();
+    buf[l] = v;
+    emu::sync_wave();
+    int base = (l / width) * width;
+    T r = (src >= 0 && src < width) ? buf[base + src] : v;
+    emu::sync_wave();
+    return r;
+}
+
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 // ---- MFMA emulation (wave64; operand/result layouts of the 16x16 instructions) -------------------
