@@ -4,7 +4,8 @@ Drop-in for the reference package's API on this path: ``dense`` tensors with per
 encoder and format; ``einsum``; ``join_legs`` / ``split_legs``; ``svd`` / ``eig`` with '(ij)(kl)'
 grouping; ``hconjugate``; ``sqrt`` / ``power``; ``param.gparity / sgn / encoder``; ``block`` tensors
 (with the block truncation rule); and the coarse-graining drivers ``trg``, ``atrg2dy``, ``atrg2dx``,
-``hotrg3dz``, ``logZ``, ``logZhotrg3dz``, ``zcap``.  Numeric work runs in
+``hotrg3dz``, ``logZ``, ``logZhotrg3dz``, ``zcap``; ``save_state`` / ``load_state`` checkpoint a
+coarse-graining run.  Numeric work runs in
 the HIP kernels of libgtn_hip.so (C ABI: include/gtn_hip.h); there is no CPU fallback.
 """
 from . import param
@@ -14,6 +15,7 @@ from .core import (dense, sparse, einsum, join_legs, split_legs, svd, eig, svd_m
                    is_grassmann_even, precision, settings)
 from .block import block, zero_block, zero_block_eo, einsum_block, svd_block, eig_block, hconjugate_block
 from .gauge2d import zcap, logZ, trg, atrg2dy, atrg2dx, hotrg3dz, logZhotrg3dz
+from .state import save_state, load_state
 
 hybrid_symbol = "*"
 separator_list = ("|", ":", ";", ",", ".")
