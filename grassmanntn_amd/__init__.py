@@ -2,7 +2,7 @@
 
 Drop-in for the reference package's API on this path: ``dense`` tensors with per-leg statistics,
 encoder and format; ``einsum``; ``join_legs`` / ``split_legs``; ``svd`` / ``eig`` with '(ij)(kl)'
-grouping; ``hconjugate``; ``sqrt`` / ``power``; ``param.gparity / sgn / encoder``; ``block`` tensors
+grouping; ``hconjugate``; ``sqrt`` / ``power`` / ``inv_nz`` / ``exp`` / ``pinv``; ``param.gparity / sgn / encoder``; ``block`` tensors
 (with the block truncation rule); and the coarse-graining drivers ``trg``, ``atrg2dy``, ``atrg2dx``,
 ``hotrg3dz``, ``logZ``, ``logZhotrg3dz``, ``zcap``; ``save_state`` / ``load_state`` checkpoint a
 coarse-graining run.  Numeric work runs in
@@ -12,7 +12,7 @@ from . import param
 from ._lib import GrassmannDeviceError
 from ._plan import GrassmannError, denumerate
 from .core import (dense, sparse, einsum, join_legs, split_legs, svd, eig, svd_many, hconjugate, power, sqrt,
-                   is_grassmann_even, precision, settings)
+                   inv_nz, exp, pinv, is_grassmann_even, precision, settings)
 from .block import block, zero_block, zero_block_eo, einsum_block, svd_block, eig_block, hconjugate_block
 from .gauge2d import zcap, logZ, trg, atrg2dy, atrg2dx, hotrg3dz, logZhotrg3dz
 from .state import save_state, load_state

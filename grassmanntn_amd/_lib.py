@@ -104,6 +104,7 @@ def _bind(lib):
         "gtn_reduce_last": (C.c_int, [vp, vp, i64, i64, i32, vp]),
         "gtn_lognorm_update": (C.c_int, [vp, vp, vp]),
         "gtn_power_diag": (C.c_int, [vp, i64, i64, i32, d, d, vp]),
+        "gtn_elementwise": (C.c_int, [vp, i64, i32, i32, d, d, vp]),
         "gtn_axpby": (C.c_int, [vp, vp, vp, i64, i32, d, d, d, d, vp]),
         "gtn_odd_norm": (C.c_int, [vp, i32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32), i32, vp, vp]),
         "gtn_svd_workspace_bytes": (C.c_int64, [C.POINTER(SvdBlock), i32, C.POINTER(SvdOpts)]),

@@ -1356,3 +1356,57 @@ def power(T, p, rcond=1e-10):
 
 def sqrt(T, rcond=1e-10):
     return power(T, 0.5)
+
+
+def _elementwise(T, op, rcond):
+    """x -> (|x| > rcond) ? f(x) : 0 on the matrix-format canonical coefficients (the common shape of
+    power_ds / inv_nz_ds / exp_ds, __init__.py:5992-6106); blocks act on their storage."""
+    from .block import block
+    if isinstance(T, block):
+        fmt = T.format
+        B = T.force_format("matrix")
+        t = B._t.copy()
+        L.check(L.lib().gtn_elementwise(L.ptr(t.data), t.data.numel(), L.dtype_code(t.data), op, 0.0, float(rcond),
+                                        L.stream_ptr()), "gtn_elementwise")
+        return block._wrap(t, B.even_shape, B.odd_shape, B.shape, B.marked_as_joined).force_format(fmt)
+    fmt, enc = T.format, T.encoder
+    M = T.force_format("matrix").force_encoder("canonical")
+    if M is T:
+        M = M.copy()
+    L.check(L.lib().gtn_elementwise(L.ptr(M.data), M.data.numel(), L.dtype_code(M.data), op, 0.0, float(rcond),
+                                    L.stream_ptr()), "gtn_elementwise")
+    M._even = None
+    return M.force_format(fmt).force_encoder(enc)
+
+
+def inv_nz(T, rcond=1e-10):
+    """Reciprocal of every coefficient above rcond, zero elsewhere (inv_nz, __init__.py:5987-6019)."""
+    return _elementwise(T, 1, rcond)
+
+
+def exp(T, rcond=1e-10):
+    """exp of every coefficient above rcond, zero elsewhere (exp_ds / exp_block, __init__.py:6086-6120);
+    Grassmann numbers (arith) are not provided by this build."""
+    return _elementwise(T, 2, rcond)
+
+
+def pinv(InpObj, string, rcond=1e-10):
+    """Pseudo-inverse through the SVD (pinv, __init__.py:5961-5983): V^+ inv_nz(S^+) U^+ with the
+    legs of the right group first."""
+    string = denumerate(string.replace(" ", ""))
+    stra = strb = None
+    for sep in SEPARATORS:
+        if sep in string:
+            stra, strb = string.split(sep)
+            break
+    if stra is None:
+        fail("Error[pinv]: the leg grouping needs a separator such as '|'.")
+    strba = strb + stra
+    strx = fresh_char(strba)
+    stry = fresh_char(strba + strx)
+    U, S, V = InpObj.svd(string)
+    Uinv = U.hconjugate(stra + "|" + strx)
+    Sinv = inv_nz(S.hconjugate(strx + "|" + stry), rcond=rcond)
+    Vinv = V.hconjugate(strx + "|" + strb)
+    ret = einsum(strb + strx + "," + strx + stry + "->" + strb + stry, Vinv, Sinv)
+    return einsum(strb + stry + "," + stry + stra + "->" + strba, ret, Uinv)

@@ -531,6 +531,16 @@ __global__ void k_power_diag(T *x, int64_t n, int64_t stride, double p, double r
     }
 }
 
+template <typename T>
+__global__ void k_elementwise(T *x, int64_t n, int op, double p, double rcond) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)x[i];
+        double r = 0.0;
+        if (fabs(v) > rcond) r = (op == GTN_EW_INV) ? 1.0 / v : (op == GTN_EW_EXP ? exp(v) : pow(v, p));
+        x[i] = (T)r;
+    }
+}
+
 struct OddArgs {
     int32_t ndim;
     uint32_t dims[GTN_PERM_MAX_AX];
@@ -977,6 +987,21 @@ int gtn_power_diag(void *x, int64_t n, int64_t stride, int dtype, double p, doub
         case GTN_F64: hipLaunchKernelGGL(k_power_diag<double>, g, b, 0, s, (double *)x, n, stride, p, rcond); break;
         case GTN_F32: hipLaunchKernelGGL(k_power_diag<float>, g, b, 0, s, (float *)x, n, stride, p, rcond); break;
         default: set_error("gtn_power_diag: unsupported dtype"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_elementwise(void *x, int64_t n, int dtype, int op, double p, double rcond, void *stream) {
+    GTN_REQUIRE(n >= 0 && (n == 0 || x), "gtn_elementwise: bad buffer");
+    GTN_REQUIRE(op == GTN_EW_POWER || op == GTN_EW_INV || op == GTN_EW_EXP, "gtn_elementwise: unknown op %d", op);
+    if (n == 0) return GTN_OK;
+    hipStream_t s = as_stream(stream);
+    dim3 g(grid_for(n, 256)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_elementwise<double>, g, b, 0, s, (double *)x, n, op, p, rcond); break;
+        case GTN_F32: hipLaunchKernelGGL(k_elementwise<float>, g, b, 0, s, (float *)x, n, op, p, rcond); break;
+        default: set_error("gtn_elementwise: unsupported dtype"); return GTN_ENOTSUP;
     }
     GTN_CHECK_LAUNCH();
     return GTN_OK;

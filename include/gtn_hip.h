@@ -175,6 +175,9 @@ int gtn_mfma_selftest(double *out_dev, void *stream);
  * gtn_lognorm_update: (log-norm accumulation of example.py:192) lognorm[0] = 2*lognorm[0] + 0.5*log(nrm2[0]).
  * gtn_power_diag: matrix-format power of a diagonal with the reference's |x| > rcond rule
  *   (power_ds, __init__.py:6059-6069): x = (|x| > rcond) ? x^p : 0, elementwise on n elements.
+ * gtn_elementwise: x = (|x| > rcond) ? f(x) : 0 elementwise on n elements, f by op: GTN_EW_POWER
+ *   x^p (power_ds), GTN_EW_INV 1/x (inv_nz_ds, __init__.py:5992-6004), GTN_EW_EXP exp(x) (exp_ds,
+ *   __init__.py:6093-6106); the matrix-format canonical coefficients are what the caller passes.
  * gtn_axpby: out = alpha * x + beta * y elementwise (dense arithmetic, __init__.py:933-987); y may be
  *   NULL (then out = alpha * x); complex tensors take complex alpha/beta.
  * gtn_odd_norm: out_dev[0] = sum |x|^2 over entries whose fermionic parity is odd (is_grassmann_even,
@@ -185,6 +188,10 @@ int gtn_scale_by_norm(void *x_dev, int64_t n, int dtype, const double *nrm2_dev,
 int gtn_reduce_last(const void *src_dev, void *dst_dev, int64_t rows, int64_t inner, int dtype, void *stream);
 int gtn_lognorm_update(double *lognorm_dev, const double *nrm2_dev, void *stream);
 int gtn_power_diag(void *x_dev, int64_t n, int64_t stride, int dtype, double p, double rcond, void *stream);
+#define GTN_EW_POWER 0
+#define GTN_EW_INV 1
+#define GTN_EW_EXP 2
+int gtn_elementwise(void *x_dev, int64_t n, int dtype, int op, double p, double rcond, void *stream);
 int gtn_axpby(const void *x_dev, const void *y_dev, void *out_dev, int64_t n, int dtype, double alpha_re,
               double alpha_im, double beta_re, double beta_im, void *stream);
 int gtn_odd_norm(const void *x_dev, int ndim, const int64_t *dims, const int32_t *fermionic,

@@ -948,6 +948,40 @@ def sqrt(T):
     return power(T, 0.5)
 
 
+def _elementwise(T, f, rcond):
+    """(|x| > rcond) ? f(x) : 0 on the matrix-format canonical coefficients (__init__.py:5992-6106)."""
+    fmt, enc = T.format, T.encoder
+    M = T.force_format("matrix").force_encoder("canonical")
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        M.data = np.where(np.abs(M.data) > rcond, f(M.data.astype(np.result_type(M.data, float))), 0.0)
+    return M.force_format(fmt).force_encoder(enc)
+
+
+def inv_nz(T, rcond=1e-10):
+    """inv_nz_ds (__init__.py:5992-6004)."""
+    return _elementwise(T, lambda x: 1.0 / x, rcond)
+
+
+def exp(T, rcond=1e-10):
+    """exp_ds (__init__.py:6093-6106)."""
+    return _elementwise(T, np.exp, rcond)
+
+
+def pinv(T, string, rcond=1e-10):
+    """pinv (__init__.py:5961-5983): V^+ inv_nz(S^+) U^+, legs of the right group first."""
+    stra, strb = string.split("|")
+    strba = strb + stra
+    used = set(strba)
+    free = [c for c in "xyzwvutsrqponmlkjihgfedcbaXYZ" if c not in used]
+    strx, stry = free[0], free[1]
+    U, S, V = svd(T, string)
+    Uinv = hconjugate(U, stra + "|" + strx)
+    Sinv = inv_nz(hconjugate(S, strx + "|" + stry), rcond=rcond)
+    Vinv = hconjugate(V, strx + "|" + strb)
+    ret = einsum(strb + strx + "," + strx + stry + "->" + strb + stry, Vinv, Sinv)
+    return einsum(strb + stry + "," + stry + stra + "->" + strba, ret, Uinv)
+
+
 # --------------------------------------------------------------------------------------------------
 # A15-A17: TRG, ATRG, logZ, zcap (gauge2d.py:1591-1885) and the driver's accumulation (example.py:162-196)
 # --------------------------------------------------------------------------------------------------

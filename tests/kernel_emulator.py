@@ -308,6 +308,16 @@ class FakeLib:
             v[::stride] = np.where(np.abs(w) > rcond, np.power(w.astype(np.float64), p), 0.0)
         return 0
 
+    def gtn_elementwise(self, x, n, dtype, op, p, rcond, stream):
+        if n == 0:
+            return 0
+        v = self._arr(x, n, NP[dtype], "elementwise")
+        w = v.astype(np.float64)
+        with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+            f = 1.0 / w if op == 1 else (np.exp(w) if op == 2 else np.power(w, p))
+            v[:] = np.where(np.abs(w) > rcond, f, 0.0)
+        return 0
+
     def gtn_axpby(self, x, y, out, n, dtype, ar, ai, br, bi, stream):
         dt = NP[dtype]
         a = complex(ar, ai) if dtype == 2 else ar

@@ -118,3 +118,26 @@ def test_hotrg3dz_literal_equals_sector(rng):
     b, nb = O.hotrg3dz(T1, T2, dcut=6, mode="sector")
     assert abs(na - nb) < 1e-12 * na
     assert abs(O.logZhotrg3dz(a, a) - O.logZhotrg3dz(b, b)) < 1e-10
+
+
+def test_pinv_moore_penrose(rng):
+    """pinv (__init__.py:5961-5983) satisfies A P A = A and P A P = P under the Grassmann einsum."""
+    A = random_even(rng, (4, 4, 8, 8), (1, 1, -1, -1), "parity-preserving")
+    P = O.pinv(A, "ab|cd")
+    assert P.shape == (8, 8, 4, 4)
+    scale = np.abs(A.data).max()
+    assert np.abs(O.einsum("abcd,cdef,efgh->abgh", A, P, A).data - A.data).max() < 1e-12 * scale
+    assert np.abs(O.einsum("cdab,abef,efgh->cdgh", P, A, P).data - P.data).max() < 1e-12 * np.abs(P.data).max()
+
+
+def test_inv_nz_and_exp_identities(rng):
+    """inv_nz is an involution on the entries above rcond, and inv_nz(exp(T)) = exp(-T) there (the
+    masks agree: exp(x) != 0 exactly where |x| > rcond); both keep the format and encoder."""
+    A = random_even(rng, (4, 8, 4), (1, -1, 1), "parity-preserving")
+    kept = np.where(np.abs(A.data) > 1e-10, A.data, 0.0)
+    twice = O.inv_nz(O.inv_nz(A))
+    assert (twice.format, twice.encoder) == (A.format, A.encoder)
+    assert np.abs(twice.data - kept).max() < 1e-13
+    lhs = O.inv_nz(O.exp(A)).data
+    rhs = O.exp(A * -1.0).data
+    assert np.abs(lhs - rhs).max() < 1e-13
