@@ -1,6 +1,7 @@
 """grassmanntn_amd -- the Grassmann-tensor hot path of grassmanntn on AMD MI355X (gfx950).
 
-Drop-in for the reference package's API on this path: ``dense`` tensors with per-leg statistics,
+Drop-in for the reference package's API on this path: ``dense`` (and host COO ``sparse``) tensors
+with per-leg statistics,
 encoder and format; ``einsum``; ``join_legs`` / ``split_legs``; ``svd`` / ``eig`` with '(ij)(kl)'
 grouping; ``hconjugate``; ``sqrt`` / ``power`` / ``inv_nz`` / ``exp`` / ``pinv``; ``param.gparity / sgn / encoder``; ``block`` tensors
 (with the block truncation rule); and the coarse-graining drivers ``trg``, ``atrg2dy``, ``atrg2dx``,

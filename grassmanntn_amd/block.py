@@ -66,7 +66,7 @@ class block:
             self.shape, self.marked_as_joined = o.shape, o.marked_as_joined
             return
         if isinstance(data, sparse):
-            fail("Error[block]: the sparse type is not provided by this build.")
+            data = dense(data)
         if not isinstance(data, dense):
             fail("Error[block]: a block tensor is built from a dense tensor (dense(...).toblock()).")
         T = data.force_encoder("parity-preserving")
@@ -220,7 +220,7 @@ class block:
         return t.force_encoder(encoder)
 
     def tosparse(self, encoder="canonical"):
-        fail("Error[block.tosparse]: the sparse type is not provided by this build.")
+        return sparse(self.todense(encoder))
 
     def is_even(self):
         return self._t.is_even()

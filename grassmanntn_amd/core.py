@@ -105,7 +105,8 @@ class dense:
             self.format = data.format
             self._even = data._even
         elif isinstance(data, sparse):
-            fail("Error[dense]: the sparse type is not provided by this build.")
+            self.data = _to_device(data.toarray())
+            self.statistics, self.encoder, self.format = data.statistics, data.encoder, data.format
         elif data is None:
             given = False
         else:
@@ -292,11 +293,184 @@ class dense:
 
 
 class sparse:
-    """Type placeholder so that ``type(T) == grassmanntn_amd.sparse`` checks work (the reference's
-    sparse COO tensors, __init__.py:1196-1478, are not provided by this build)."""
+    """Sparse (COO) Grassmann tensor (__init__.py:1196-1478): coordinates and values on the host,
+    with the same statistics / encoder / format properties as ``dense``.  Numeric operations upload
+    the tensor to the device as a dense tensor and return a sparse result when the reference does
+    (the type of the first operand)."""
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError("sparse Grassmann tensors are not provided by this build; use dense")
+    __array_priority__ = 1000000
+
+    def __init__(self, data=None, encoder="canonical", format="standard", statistics=None):
+        if encoder not in ENCODERS:
+            fail("Error[sparse]: Unknown encoder.")
+        if format not in FORMATS:
+            fail("Error[sparse]: Unknown format.")
+        self.statistics = None
+        self.encoder, self.format = encoder, format
+        self._c = np.zeros((0, 0), dtype=np.int64)
+        self._v = np.zeros(0)
+        self._shape = ()
+        given = True
+        from .block import block
+        if isinstance(data, block):
+            data = data.todense()
+        if isinstance(data, sparse):
+            self._c, self._v, self._shape = data._c.copy(), data._v.copy(), data._shape
+            self.statistics, self.encoder, self.format = data.statistics, data.encoder, data.format
+        elif isinstance(data, dense):
+            self._from_array(data.data.numpy())
+            self.statistics, self.encoder, self.format = data.statistics, data.encoder, data.format
+        elif isinstance(data, (np.ndarray, list)):
+            self._from_array(np.array(data))
+        elif data is None:
+            given = False
+        else:
+            fail("Error[sparse]: Invalid initialized data")
+        if statistics is not None:
+            self.statistics = tuple(statistics) if not np.isscalar(statistics) else (statistics,)
+        if given and not _pkg_flag("skip_power_of_two_check", False):
+            for i, d in enumerate(self._shape):
+                if self.statistics[i] in FERMI and d != 2 ** int(math.floor(math.log2(d))):
+                    fail("Error[sparse]: Some of the fermionic tensor shapes are not a power of two."
+                         "\n               Have you added the <statistics> argument when calling this function?")
+
+    def _from_array(self, a):
+        idx = np.nonzero(a)
+        self._c = np.array(idx, dtype=np.int64).reshape(a.ndim, -1)
+        self._v = a[idx].copy()
+        self._shape = tuple(a.shape)
+
+    def toarray(self):
+        """The coefficients as a host numpy array."""
+        a = np.zeros(self._shape, dtype=self._v.dtype if self._v.size else np.float64)
+        if self._v.size:
+            a[tuple(self._c)] = self._v
+        return a
+
+    shape = property(lambda self: self._shape)
+    ndim = property(lambda self: len(self._shape))
+    size = property(lambda self: int(np.prod(self._shape)) if self._shape else 1)
+    nnz = property(lambda self: int(self._v.size))
+    coords = property(lambda self: [tuple(int(x) for x in c) for c in self._c.T])
+    value = property(lambda self: list(self._v))
+
+    @property
+    def norm(self):
+        return float(np.linalg.norm(self._v))
+
+    def _like(self, coords, values):
+        r = sparse.__new__(sparse)
+        r._c, r._v, r._shape = coords, values, self._shape
+        r.statistics, r.encoder, r.format = self.statistics, self.encoder, self.format
+        return r
+
+    def copy(self):
+        return self._like(self._c.copy(), self._v.copy())
+
+    def remove_zeros(self):
+        keep = self._v != 0
+        return self._like(self._c[:, keep], self._v[keep])
+
+    def set_value(self, entry, value):
+        self._v[entry] = value
+
+    def remove_entry(self, entry):
+        keep = np.ones(self.nnz, dtype=bool)
+        keep[entry] = False
+        self._c, self._v = self._c[:, keep], self._v[keep]
+
+    def append_entry(self, coords, value):
+        self._c = np.concatenate([self._c, np.array(coords, dtype=np.int64).reshape(-1, 1)], axis=1)
+        self._v = np.concatenate([self._v, np.array([value], dtype=np.result_type(self._v, value))])
+
+    # arithmetic (__init__.py:1377-1405): through the dense form
+    def __add__(self, other):
+        return sparse(dense(self) + dense(other))
+
+    def __mul__(self, other):
+        if not np.isscalar(other):
+            fail("Error[sparse.*]: Only scalar multiplication is allowed.")
+        return self._like(self._c.copy(), self._v * other)
+
+    def __truediv__(self, other):
+        if not np.isscalar(other):
+            fail("Error[sparse./]: Only scalar division is allowed.")
+        return self._like(self._c.copy(), self._v / other)
+
+    def __pos__(self):
+        return self
+
+    def __neg__(self):
+        return self * (-1)
+
+    def __radd__(self, other):
+        return self + other
+
+    def __sub__(self, other):
+        return self + (-1) * other
+
+    def __rsub__(self, other):
+        return other + (-1) * self
+
+    def __rmul__(self, other):
+        return self * other
+
+    # encoder / format / structure through the device (__init__.py:1426-1478)
+    def switch_format(self, save_memory=False):
+        return sparse(dense(self).switch_format())
+
+    def switch_encoder(self, save_memory=False):
+        return sparse(dense(self).switch_encoder())
+
+    def switch_parity(self, save_memory=False):
+        return sparse(dense(self).switch_parity())
+
+    def force_encoder(self, target="canonical"):
+        return self.switch_encoder() if target != self.encoder else self.copy()
+
+    def force_format(self, target="standard"):
+        return self.switch_format() if target != self.format else self.copy()
+
+    def join_legs(self, string_inp, make_format="standard", intermediate_stat=None, save_memory=False):
+        return join_legs(self, string_inp, make_format, intermediate_stat)
+
+    def split_legs(self, string_inp, final_stat, final_shape, intermediate_stat=None, save_memory=False):
+        return split_legs(self, string_inp, final_stat, final_shape, intermediate_stat)
+
+    def hconjugate(self, input_string, save_memory=False):
+        return hconjugate(self, input_string)
+
+    def svd(self, string_inp, cutoff=None, save_memory=False):
+        return svd(self, string_inp, cutoff)
+
+    def eig(self, string_inp, cutoff=None, debug_mode=False, save_memory=False):
+        return eig(self, string_inp, cutoff)
+
+    def toblock(self):
+        from .block import block
+        return block(dense(self))
+
+    def todense(self):
+        return dense(self)
+
+    def pinv(self, string, rcond=1e-10):
+        return pinv(self, string, rcond=rcond)
+
+
+def _sparse_in(fn):
+    """Run `fn` on the dense form of a sparse first argument and return sparse results (the
+    reference's this_type == sparse branches)."""
+    import functools
+
+    @functools.wraps(fn)
+    def wrapped(T, *a, **k):
+        if not isinstance(T, sparse):
+            return fn(T, *a, **k)
+        out = fn(dense(T), *a, **k)
+        if isinstance(out, tuple):
+            return tuple(sparse(x) if isinstance(x, dense) else x for x in out)
+        return sparse(out) if isinstance(out, dense) else out
+    return wrapped
 
 
 # --------------------------------------------------------------------------------------------------
@@ -369,6 +543,11 @@ def einsum(*args, ignore_anticommutation=False):
     from .block import block, einsum_block
     if isinstance(ops[0], block):
         return einsum_block(subscripts, *ops, ignore_anticommutation=ignore_anticommutation)
+    if isinstance(ops[0], sparse):
+        # computed dense on the device; the result takes the first operand's type (__init__.py:2265)
+        res = einsum(args[0], *[dense(o) if isinstance(o, sparse) else o for o in ops],
+                     ignore_anticommutation=ignore_anticommutation)
+        return sparse(res) if isinstance(res, dense) else res
     ops = [o if isinstance(o, dense) else dense(o) for o in ops]
     first = ops[0]
     if nobj >= 3:
@@ -827,6 +1006,7 @@ def _intermediate(srt, inter):
     return res
 
 
+@_sparse_in
 def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, save_memory=False):
     """Fuse groups of legs into one leg each; the result is parity-preserving (__init__.py:2947-3068).
     Bosons move left inside a group, +1 legs in a -1 group pick up (-1)^p, the fermionic legs fuse in
@@ -901,6 +1081,7 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
     return out
 
 
+@_sparse_in
 def split_legs(T, string_inp, final_stat, final_shape, intermediate_stat=None, save_memory=False):
     """Inverse of join_legs (__init__.py:3070-3170), including its early return for matrix-format
     input: such input comes back canonical and in matrix format."""
@@ -1296,11 +1477,13 @@ def svd_many(calls, mode=0):
     return [r[:3] for r in _decompose_many(calls, mode)]
 
 
+@_sparse_in
 def svd(InpObj, string, cutoff=None, save_memory=False):
     """(__init__.py:4033-4308) U, S, V with T = einsum(U S V); parity blocks kept int(cutoff/2) each."""
     return _decompose(InpObj, string, cutoff, 0)
 
 
+@_sparse_in
 def eig(InpObj, string, cutoff=None, debug_mode=False, save_memory=False):
     """(__init__.py:4425-4698) U, Lambda, U^+ of a Hermitian tensor; eigenvalues ordered by |lambda|."""
     M = InpObj
@@ -1310,6 +1493,7 @@ def eig(InpObj, string, cutoff=None, debug_mode=False, save_memory=False):
 # --------------------------------------------------------------------------------------------------
 # hconjugate (__init__.py:5300-5493)
 # --------------------------------------------------------------------------------------------------
+@_sparse_in
 def hconjugate(InpObj, string, save_memory=False):
     from .block import block, hconjugate_block
     if isinstance(InpObj, block):
@@ -1342,6 +1526,7 @@ def hconjugate(InpObj, string, save_memory=False):
 # --------------------------------------------------------------------------------------------------
 # power / sqrt (__init__.py:6050-6069)
 # --------------------------------------------------------------------------------------------------
+@_sparse_in
 def power(T, p, rcond=1e-10):
     from .block import block, power_block
     if isinstance(T, block):
@@ -1379,17 +1564,20 @@ def _elementwise(T, op, rcond):
     return M.force_format(fmt).force_encoder(enc)
 
 
+@_sparse_in
 def inv_nz(T, rcond=1e-10):
     """Reciprocal of every coefficient above rcond, zero elsewhere (inv_nz, __init__.py:5987-6019)."""
     return _elementwise(T, 1, rcond)
 
 
+@_sparse_in
 def exp(T, rcond=1e-10):
     """exp of every coefficient above rcond, zero elsewhere (exp_ds / exp_block, __init__.py:6086-6120);
     Grassmann numbers (arith) are not provided by this build."""
     return _elementwise(T, 2, rcond)
 
 
+@_sparse_in
 def pinv(InpObj, string, rcond=1e-10):
     """Pseudo-inverse through the SVD (pinv, __init__.py:5961-5983): V^+ inv_nz(S^+) U^+ with the
     legs of the right group first."""

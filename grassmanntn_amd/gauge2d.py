@@ -24,6 +24,11 @@ def logZ(T, boundary_conditions="periodic"):
     complex logarithm (the reference's np.log returns nan there).  Block tensors get no anti-periodic
     signs, as in the block driver (gauge2d_block.py:1601-1615)."""
     from .block import block
+    if isinstance(T, gtn.sparse):
+        # the reference densifies a sparse tensor and takes it to the canonical encoder first
+        T = gtn.dense(T)
+        if T.encoder == "parity-preserving":
+            T = T.switch_encoder()
     if boundary_conditions == "anti-periodic" and not isinstance(T, block):
         T = T.copy()
         # sign (-1)^popcount of the stored index along leg 1, as a signed gather
@@ -232,6 +237,14 @@ def _antiperiodic_J(T):
 def logZhotrg3dz(T1, T2, boundary_conditions="periodic"):
     """ln of einsum('IJIJmn,KLKLmn') (gauge2d.py:1617-1641); anti-periodic: (-1)^popcount of the
     stored index along leg J of both tensors.  A negative real Z gives the complex logarithm."""
+    def densify(T):
+        if isinstance(T, gtn.sparse):
+            T = gtn.dense(T)
+            if T.encoder == "parity-preserving":
+                T = T.switch_encoder()
+        return T
+
+    T1, T2 = densify(T1), densify(T2)
     if boundary_conditions == "anti-periodic":
         T1, T2 = _antiperiodic_J(T1), _antiperiodic_J(T2)
     Z = gtn.einsum("IJIJmn,KLKLmn", T1, T2)

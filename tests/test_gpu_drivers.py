@@ -142,3 +142,18 @@ def test_state_checkpoint_resume_on_device(tmp_path):
     R, _, _ = gtn.load_state(tmp_path / "b.npz")
     assert (R.even_shape, R.odd_shape, tuple(R.shape)) == (B.even_shape, B.odd_shape, tuple(B.shape))
     assert np.array_equal(R.todense().data.numpy(), B.todense().data.numpy())
+
+
+def test_sparse_trg_matches_dense():
+    """A sparse (COO) site tensor through one TRG step on the device gives the dense Tnorm and ln Z
+    (the equality the reference's documentation shows, docs/_sources/schwinger.rst.txt:920-921)."""
+    T0 = O.site_tensor(16, 4)
+    D = to_dev(T0)
+    S = gtn.sparse(D)
+    Ts, ns = gtn.trg(S, 16)
+    Td, nd = gtn.trg(D, 16)
+    assert isinstance(Ts, gtn.sparse)
+    assert abs(ns - nd) <= 1e-12 * nd
+    assert abs(gtn.logZ(Ts) - gtn.logZ(Td)) < 1e-10
+    ref, nref = O.trg(T0, 16)
+    assert abs(ns - nref) <= 1e-10 * nref
