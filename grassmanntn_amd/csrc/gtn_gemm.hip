@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
     const TIN *A = (const TIN *)g.A + b * g.sab + kb * g.sak;
     const TIN *B = (const TIN *)g.B + b * g.sbb + kb * g.sbk;
     TIN *C = (TIN *)g.C + b * g.scb + sp * g.scs;
-    constexpr int PAD = 2;
+    constexpr int PAD = 1;  // rows of BM + 1: lanes storing one k-column of a row-major panel hit distinct banks
     __shared__ TC As[BK][BM + PAD];
     __shared__ TC Bs[BK][BN + PAD];
     constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
