@@ -1270,10 +1270,12 @@ def _materialize(specs):
     return res
 
 
-def _svd_blocks_raw(blocks, mode):
+def _svd_blocks_raw(blocks, mode, readback=True):
     """blocks: list of (contiguous FP64 matrix, k).  Returns (per block (U (m x k), S (k), Vt (k x n), k),
     per block the number of nonzero values in S).  Consecutive blocks of equal shape get their outputs
-    in shared buffers at constant strides."""
+    in shared buffers at constant strides.  With readback=False nothing is read back: the counts are
+    taken to be k and the third value returned, (groups, S buffers, keys), lets _Pending check them
+    later."""
     lib = L.lib()
     nblk = len(blocks)
     arr = (L.SvdBlock * nblk)()
@@ -1313,7 +1315,14 @@ def _svd_blocks_raw(blocks, mode):
     ws_bytes = lib.gtn_svd_workspace_bytes(arr, nblk, C.byref(opts))
     ws = DeviceArray.empty(max(ws_bytes // 8, 1), np.float64)
     L.check(lib.gtn_svd_blocks(arr, nblk, C.byref(opts), L.ptr(ws), ws_bytes, L.stream_ptr()), "gtn_svd_blocks")
-    # the rank rule's count of nonzero values, one read-back per group
+    if not readback:
+        return outs, [keys[i][2] for i in range(nblk)], (groups, sbufs, keys)
+    return outs, _rank_counts(groups, sbufs, keys)
+
+
+def _rank_counts(groups, sbufs, keys):
+    """The rank rule's count of nonzero values per block, one read-back per group of equal blocks."""
+    counts = [0] * len(keys)
     for (st, cnt), S in zip(groups, sbufs):
         kk = keys[st][2]
         h = S.numpy()
@@ -1322,7 +1331,22 @@ def _svd_blocks_raw(blocks, mode):
                                          "inf/nan or the decomposition broke down)")
         for j in range(cnt):
             counts[st + j] = int(np.count_nonzero(h[j][:kk])) if kk else 0
-    return outs, counts
+    return counts
+
+
+class _Pending:
+    """Checks of a decomposition run without read-back (svd_many(..., speculate=True)): the rank
+    counts assumed equal to the requested k, the finiteness of the values and the parity leaks."""
+
+    def __init__(self, raw, assumed, leaks):
+        self.raw, self.assumed, self.leaks = raw, assumed, leaks
+
+    def verify(self):
+        """True when every count was the assumed one; raises what the synchronous path raises."""
+        counts = _rank_counts(*self.raw)
+        for leak in self.leaks:
+            _check_parity_leak(*leak)
+        return counts == self.assumed
 
 
 class _Job:
@@ -1446,22 +1470,29 @@ def _decompose_finish(job, outs, counts):
     return tuple(out)
 
 
-def _decompose_many(calls, mode, rule="dense"):
+def _decompose_many(calls, mode, rule="dense", speculate=False):
     """Several svd / eig calls [(T, string, cutoff), ...] with all their parity blocks decomposed in
-    one gtn_svd_blocks call.  Returns [(U, S, V, kept), ...] (kept: the larger per-parity count)."""
+    one gtn_svd_blocks call.  Returns [(U, S, V, kept), ...] (kept: the larger per-parity count).
+    speculate=True assumes every block keeps its requested count instead of reading the counts back
+    (no synchronisation) and returns (results, _Pending) for the caller to verify."""
     jobs = [_decompose_prepare(T, string, cutoff, mode, rule) for T, string, cutoff in calls]
     specs = [b for j in jobs for b in j.blocks]
-    outs, counts = _svd_blocks_raw(_materialize(specs), mode) if specs else ([], [])
-    for j in jobs:
-        if j.leak is not None:
-            _check_parity_leak(*j.leak)
+    pending = None
+    if speculate and specs:
+        outs, counts, raw = _svd_blocks_raw(_materialize(specs), mode, readback=False)
+        pending = _Pending(raw, list(counts), [j.leak for j in jobs if j.leak is not None])
+    else:
+        outs, counts = _svd_blocks_raw(_materialize(specs), mode) if specs else ([], [])
+        for j in jobs:
+            if j.leak is not None:
+                _check_parity_leak(*j.leak)
     res, at = [], 0
     for j in jobs:
         nb = len(j.blocks)
         r = j.done if j.done is not None else _decompose_finish(j, outs[at:at + nb], counts[at:at + nb])
         at += nb
         res.append(tuple(r) + (j.kept[0] if j.kept else None,))
-    return res
+    return (res, pending) if speculate else res
 
 
 def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
@@ -1471,9 +1502,14 @@ def _decompose(T, string, cutoff, mode, rule="dense", kept=None):
     return U, S, V
 
 
-def svd_many(calls, mode=0):
+def svd_many(calls, mode=0, speculate=False):
     """[(T, string, cutoff), ...] -> [(U, S, V), ...]: several dense svd calls (eig with mode=1) whose
-    parity blocks share one batched device decomposition."""
+    parity blocks share one batched device decomposition.  speculate=True returns (results, pending):
+    the results assume every parity block keeps its requested number of values, and pending.verify()
+    (which synchronises) says whether that held; when it did not the results must be recomputed."""
+    if speculate:
+        res, pending = _decompose_many(calls, mode, speculate=True)
+        return [r[:3] for r in res], pending
     return [r[:3] for r in _decompose_many(calls, mode)]
 
 

@@ -51,8 +51,13 @@ def logZ(T, boundary_conditions="periodic"):
     return np.log(Z)
 
 
-def trg(T, dcut=64, iternum=None, error_test=False):
-    """One Levin-Nave TRG step (gauge2d.py:1647-1755): returns (T', ||T'||[, err])."""
+def trg(T, dcut=64, iternum=None, error_test=False, _speculate=True):
+    """One Levin-Nave TRG step (gauge2d.py:1647-1755): returns (T', ||T'||[, err]).
+
+    The step runs without waiting for the SVD's rank counts: it assumes every parity block keeps its
+    int(dcut/2) values, so that the contractions queue behind the decomposition, and checks the
+    counts (and the finiteness and parity checks of the SVD) when it reads the norm back; if a block
+    kept fewer values the step is recomputed with the read-back counts."""
     if [T.shape[0], T.shape[1]] != [T.shape[2], T.shape[3]]:
         fail("Error[trg]: The shape must be of the form (m,n,m,n)!")
     if list(T.statistics) != [1, 1, -1, -1]:
@@ -64,9 +69,14 @@ def trg(T, dcut=64, iternum=None, error_test=False):
     else:
         T1 = gtn.einsum("ijkl->jkli", T)
         T2 = gtn.einsum("ijkl->klij", T)
+    pending = None
     if isinstance(T1, gtn.dense):
         # both decompositions' parity blocks in one batched device call
-        (U1, S1, V1), (U2, S2, V2) = gtn.svd_many([(T1, "ab|cd", dcut), (T2, "ab|cd", dcut)])
+        calls = [(T1, "ab|cd", dcut), (T2, "ab|cd", dcut)]
+        if _speculate:
+            ((U1, S1, V1), (U2, S2, V2)), pending = gtn.svd_many(calls, speculate=True)
+        else:
+            (U1, S1, V1), (U2, S2, V2) = gtn.svd_many(calls)
     else:
         U1, S1, V1 = T1.svd("ab|cd", dcut)
         U2, S2, V2 = T2.svd("ab|cd", dcut)
@@ -79,6 +89,8 @@ def trg(T, dcut=64, iternum=None, error_test=False):
     VV = gtn.einsum("kwz,lxw->lxzk", V1, V2)
     UU = gtn.einsum("yxi,zyj->jzxi", U1, U2)
     T2n = gtn.einsum("lxzk,jzxi->ijkl", VV, UU)
+    if pending is not None and not pending.verify():
+        return trg(T, dcut, iternum, error_test, _speculate=False)
     err = None
     if error_test:
         Z1 = gtn.einsum("ijkl,klij", T, T)
