@@ -33,6 +33,21 @@ def random_even(rng, shape, stats, encoder="canonical"):
     return T
 
 
+def diagonal_site(D, seed, terms=3):
+    """T = sum_n c_n e(a_n, b_n, a_n, b_n) over `terms` distinct pairs of even indices, an oracle dense
+    tensor with statistics (1, 1, -1, -1): both TRG matrices ('jk|li' and 'kl|ij') are diagonal, with
+    `terms` nonzero elements in their even parity block and none in the odd one, so each SVD keeps
+    `terms` values exactly (no rounding noise to resolve)."""
+    from oracle import gtn_oracle as O
+    rng = np.random.default_rng(seed)
+    pairs = rng.choice((D // 2) ** 2, terms, replace=False)
+    T = np.zeros((D,) * 4)
+    for n, q in enumerate(pairs):
+        a, b = 2 * (q // (D // 2)), 2 * (q % (D // 2))
+        T[a, b, a, b] = 1.0 - 0.25 * n
+    return O.dense(T, encoder="parity-preserving", statistics=(1, 1, -1, -1))
+
+
 @pytest.fixture
 def rng():
     return np.random.default_rng(1234)

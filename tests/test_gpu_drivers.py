@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import gtn_oracle as O
-from conftest import random_even
+from conftest import diagonal_site, random_even
 
 pytestmark = pytest.mark.gpu
 
@@ -159,25 +159,11 @@ def test_sparse_trg_matches_dense():
     assert abs(ns - nref) <= 1e-10 * nref
 
 
-def _diagonal_site(D, seed, terms=3):
-    """T = sum_n c_n e(a_n, b_n, a_n, b_n) over `terms` distinct pairs of even indices: both TRG
-    matrices ('jk|li' and 'kl|ij') are diagonal, with `terms` nonzero elements in their even parity
-    block and none in the odd one, so each SVD keeps `terms` values exactly (no rounding noise to
-    resolve) where int(dcut/2) were requested."""
-    rng = np.random.default_rng(seed)
-    pairs = rng.choice((D // 2) ** 2, terms, replace=False)
-    T = np.zeros((D,) * 4)
-    for n, q in enumerate(pairs):
-        a, b = 2 * (q // (D // 2)), 2 * (q % (D // 2))
-        T[a, b, a, b] = 1.0 - 0.25 * n
-    return O.dense(T, encoder="parity-preserving", statistics=(1, 1, -1, -1))
-
-
 def test_trg_rank_deficient_blocks():
     """Blocks that keep fewer values than int(dcut/2) (the zero odd blocks keep none): the step's
     assumed counts fail its check and it is recomputed with the read-back counts; shape, norm and
     ln Z match the oracle."""
-    T0 = _diagonal_site(8, 7)
+    T0 = diagonal_site(8, 7)
     ref, nref = O.trg(T0, 8)
     got, nrm = gtn.trg(to_dev(T0), 8)
     assert tuple(got.shape) == tuple(ref.shape), (got.shape, ref.shape)
