@@ -8,10 +8,11 @@ one gtn.trg call (two signed permutations, two parity-block SVDs, six contractio
 
 Prints one JSON line: value = steps/s of the mixed-precision path (the first three steps all FP64,
 then contractions in split BF16 (three bf16 terms per operand) with FP32 accumulation and the SVD
-range finder in FP32; SVD projections, normalisation and the log-norm sum in FP64), with the FP64
-path's steps/s, ln Z per site of both against the FP64 CPU oracle, the roofline of the T' contraction
-GEMM (time from HIP events around each launch; HBM bytes from the committed rocprofv3 counter
-summary) and the CPU baseline (the numpy oracle timed for one step on the host).
+range finder in FP32 with 18 subspace iterations; SVD projections, normalisation and the log-norm
+sum in FP64), with the FP64 path's steps/s (20 subspace iterations), ln Z per site of both against
+the FP64 CPU oracle, the roofline of the T' contraction GEMM (time from HIP events around each
+launch; HBM bytes from the committed rocprofv3 counter summary) and the CPU baseline (the numpy
+oracle timed for one step on the host).
 """
 import argparse
 import json
@@ -146,6 +147,9 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
     ap.add_argument("--svd-iters", type=int, default=0, help="subspace iterations of the block SVD "
                     "(0: the library default)")
+    ap.add_argument("--mixed-svd-iters", type=int, default=18, help="subspace iterations of the block "
+                    "SVD in the mixed path (0: as the FP64 path); 18 keeps the mixed path's |d lnZ/site| "
+                    "where 20 puts it (3e-9, with the FP32 range finder)")
     ap.add_argument("--svd-orth-every", type=int, default=0, help="re-orthonormalise the range-finder "
                     "basis every this many iterations (0: the library default)")
     ap.add_argument("--svd-oversample", type=int, default=-1, help="range-finder columns beyond the kept "
@@ -168,7 +172,11 @@ def main():
     if args.svd_oversample >= 0:
         gtn.settings.svd_oversample = args.svd_oversample
     T0 = site_tensor(D, SEED)
+    fp64_iters = gtn.settings.svd_iters
+    mixed_iters = args.mixed_svd_iters or fp64_iters
+    gtn.settings.svd_iters = mixed_iters
     mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0, args.fp64_steps)
+    gtn.settings.svd_iters = fp64_iters
     fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "trg_golden.json")
@@ -198,7 +206,7 @@ def main():
             args.precision, args.svd_precision, args.fp64_steps),
         "data": "synthetic",
         "config": {"workload": "TRG, leg dim 64, chi 64, %d steps, seed %d" % (n_total, SEED),
-                   "leg_dim": D, "chi": CHI, "steps_total": n_total, "svd_iters": gtn.settings.svd_iters,
+                   "leg_dim": D, "chi": CHI, "steps_total": n_total, "svd_iters": mixed_iters, "svd_iters_fp64_path": fp64_iters,
                    "svd_oversample": gtn.settings.svd_oversample,
                    "svd_orth_every": gtn.settings.svd_orth_every},
         "lnz_per_site_mixed": [mixed["lnz"].real, mixed["lnz"].imag],
