@@ -48,6 +48,31 @@ def site_tensor(d, seed):
 VERBOSE = False
 
 
+def einsum_microbench(gtn, prec, reps=10):
+    """The standalone contraction 'ijkl,klmn->ijmn' of two seeded Grassmann-even 64^4 FP64 site
+    tensors (statistics (1,1,-1,-1), parity-preserving): a 4096 x 4096 x 4096 contraction after the
+    sign application, whose two nonzero parity blocks are 2048^3 GEMMs.  Mean over `reps` calls of
+    the whole einsum (operand gathers with the signs, GEMMs, output gather), timed with HIP events."""
+    from grassmanntn_amd._device import Event, synchronize
+    st = (1, 1, -1, -1)
+    A = gtn.dense(site_tensor(D, SEED + 1), encoder="parity-preserving", statistics=st)
+    B = gtn.dense(site_tensor(D, SEED + 2), encoder="parity-preserving", statistics=st)
+    with gtn.precision(prec, "f64"):
+        for _ in range(2):
+            C = gtn.einsum("ijkl,klmn->ijmn", A, B)
+        synchronize()
+        e0, e1 = Event(), Event()
+        e0.record()
+        for _ in range(reps):
+            C = gtn.einsum("ijkl,klmn->ijmn", A, B)
+        e1.record()
+        synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    flop = 2 * 2 * float(D ** 2 // 2) ** 3  # two parity blocks, 2 M N K each
+    del C
+    return {"precision": prec, "ms": ms, "flop": flop, "tflops": flop / ms / 1e9 if ms > 0 else None, "reps": reps}
+
+
 def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
     """warmup + steps TRG steps; step i runs its contractions in FP64 when i < fp64_steps, else in
     `prec` (range finder in `svd_prec`).  Only the last `steps` are timed."""
@@ -148,6 +173,8 @@ def main():
                          "weigh 2^-n in ln Z per site)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-einsum", action="store_true", help="skip the standalone 'ijkl,klmn->ijmn' "
+                    "contraction of two 64^4 tensors")
     ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
     ap.add_argument("--svd-iters", type=int, default=0, help="subspace iterations of the block SVD "
                     "(0: the library default)")
@@ -244,6 +271,8 @@ def main():
         out["dlnz_mixed_vs_oracle"] = abs(mixed["lnz"] - golden)
         if fp64 is not None:
             out["dlnz_fp64_vs_oracle"] = abs(fp64["lnz"] - golden)
+    if not args.no_einsum:
+        out["einsum_ijkl_klmn_d64"] = [einsum_microbench(gtn, "f64"), einsum_microbench(gtn, args.precision)]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
