@@ -13,7 +13,7 @@ from . import param
 from ._lib import GrassmannDeviceError
 from ._plan import GrassmannError, denumerate
 from .core import (dense, sparse, einsum, join_legs, split_legs, svd, eig, svd_many, hconjugate, power, sqrt,
-                   inv_nz, exp, pinv, is_grassmann_even, precision, settings)
+                   inv_nz, exp, pinv, is_grassmann_even, precision, settings, mixed_schedule)
 from .block import block, zero_block, zero_block_eo, einsum_block, svd_block, eig_block, hconjugate_block
 from .gauge2d import zcap, logZ, trg, atrg2dy, atrg2dx, hotrg3dz, logZhotrg3dz
 from .state import save_state, load_state

@@ -588,6 +588,40 @@ __global__ void k_sigma(const double *lam, int k, int eig_mode, double *S, doubl
     inv[i] = kp ? 1.0 / fabs(s) : 0.0;
     keep[i] = kp ? 1.0 : 0.0;
 }
+// Singular values from the unnormalised singular vectors: vector i of X (k vectors of `len`
+// elements, element j of vector i at X[i * sv + j * se]) is sigma_i times a unit vector, so
+// sigma_i = ||x_i|| carries FP64 relative accuracy down to ~1e-16 sigma_0 (sqrt of a Gram eigenvalue
+// cannot resolve sigma below ~1e-8 sigma_0).  Then the rank rule of __init__.py:3939-3941 as in
+// k_sigma.  One workgroup of 256 threads (four wavefronts) per block; wavefront w reduces vectors
+// i = w, w + 4, ...
+__global__ void __launch_bounds__(256) k_sigma_norms(const double *X, int k, int64_t len, int64_t sv, int64_t se,
+                                                     double *S, double *inv, double *keep, int64_t dX, int64_t dS,
+                                                     int64_t dw) {
+    X += blockIdx.y * dX;
+    S += blockIdx.y * dS;
+    inv += blockIdx.y * dw;
+    keep += blockIdx.y * dw;
+    __shared__ double sh[256];  // k <= 256 (checked on the host)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = w; i < k; i += 4) {
+        double acc = 0.0;
+        for (int64_t j = lane; j < len; j += 64) {
+            double x = X[i * sv + j * se];
+            acc = fma(x, x, acc);
+        }
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) sh[i] = sqrt(acc);
+    }
+    __syncthreads();
+    const double s0 = sh[0];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        double s = sh[i];
+        bool kp = fabs(s / (s0 + 1e-14)) > 1e-14;
+        S[i] = (s != s) ? s : (kp ? s : 0.0);
+        inv[i] = kp ? 1.0 / s : 0.0;
+        keep[i] = kp ? 1.0 : 0.0;
+    }
+}
 __global__ void k_scale_rows(double *X, int k, int64_t n, int64_t ld, const double *f, int64_t dX, int64_t df) {
     X += blockIdx.y * dX;
     f += blockIdx.y * df;
@@ -1131,6 +1165,7 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
     for (Run &r : ra) {
         Blk &f = *r.f;
         GTN_REQUIRE(f.b.k <= 256 && f.b.k <= f.p, "gtn_svd_blocks: k too large");
+        if (!eig) continue;  // svd: the values come from the vectors' norms below
         R8N(f.lam, f.b.k, r.dws, r.n, "lambda");
         R8N(f.b.S, f.b.k, r.dS, r.n, "S");
         R8N(f.inv, f.b.k, r.dws, r.n, "inv");
@@ -1138,7 +1173,8 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
         LAUNCH(k_sigma, dim3(1, r.n), dim3(256), 0, s, (const double *)f.lam, f.b.k, eig, f.b.S, f.inv, f.keep, r.dws, r.dS,
                r.dws);
     }
-    // ---- singular vectors
+    // ---- singular vectors (svd: unnormalised, sigma_i times the unit vector, on the side computed
+    // by a product with the block)
     std::vector<gtn_gemm_desc> g;
     for (Run &r : ra) {
         Blk &f = *r.f;
@@ -1169,6 +1205,26 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
     }
     rc = run(g, GTN_PREC_F64, s);
     if (rc) return rc;
+    if (!eig) {
+        // sigma_i = norm of the unnormalised vector i: row i of Vt, or column i of U = M V
+        for (Run &r : ra) {
+            Blk &f = *r.f;
+            const gtn_svd_block &B = f.b;
+            const int64_t k = B.k;
+            R8N(f.b.S, k, r.dS, r.n, "S");
+            R8N(f.inv, k, r.dws, r.n, "inv");
+            R8N(f.keep, k, r.dws, r.n, "keep");
+            if (f.small && !f.left) {
+                R8N(B.U, ext(B.m, k, B.ldu), r.dU, r.n, "U");
+                LAUNCH(k_sigma_norms, dim3(1, r.n), dim3(256), 0, s, (const double *)B.U, (int)k, B.m, (int64_t)1, B.ldu,
+                       B.S, f.inv, f.keep, r.dU, r.dS, r.dws);
+            } else {
+                R8N(B.Vt, ext(k, B.n, B.ldv), r.dV, r.n, "Vt");
+                LAUNCH(k_sigma_norms, dim3(1, r.n), dim3(256), 0, s, (const double *)B.Vt, (int)k, B.n, B.ldv, (int64_t)1,
+                       B.S, f.inv, f.keep, r.dV, r.dS, r.dws);
+            }
+        }
+    }
     for (Run &r : ra) {
         Blk &f = *r.f;
         const gtn_svd_block &B = f.b;

@@ -7,6 +7,18 @@ from . import core as gtn
 from .core import fail, FERMI
 
 
+class _TrgCounters:
+    """Process-wide counts of TRG steps run (`steps`) and of steps recomputed because an assumed
+    rank count did not hold (`recomputed`; `misses` lists (step number, assumed counts, read-back
+    counts) of the last few)."""
+    steps = 0
+    recomputed = 0
+    misses = []
+
+
+trg_counters = _TrgCounters()
+
+
 def zcap(T):
     """Trace the two bosonic flavour legs with an all-ones capper (gauge2d.py:1591-1597,
     gauge2d_block.py:1590-1599)."""
@@ -90,7 +102,10 @@ def trg(T, dcut=64, iternum=None, error_test=False, _speculate=True):
     UU = gtn.einsum("yxi,zyj->jzxi", U1, U2)
     T2n = gtn.einsum("lxzk,jzxi->ijkl", VV, UU)
     if pending is not None and not pending.verify():
+        trg_counters.recomputed += 1
+        trg_counters.misses = (trg_counters.misses + [(trg_counters.steps + 1, pending.assumed, pending.counts)])[-8:]
         return trg(T, dcut, iternum, error_test, _speculate=False)
+    trg_counters.steps += 1
     err = None
     if error_test:
         Z1 = gtn.einsum("ijkl,klij", T, T)

@@ -48,6 +48,29 @@ def diagonal_site(D, seed, terms=3):
     return O.dense(T, encoder="parity-preserving", statistics=(1, 1, -1, -1))
 
 
+def device_sign_tensor(plan, which, chars, shape):
+    """The sign tensor S1 (which=1) or S3 (which=3) of an oracle EinsumPlan over the axes `chars`,
+    generated on the device by gtn_sign_tensor from the plan's per-parity-pattern signs (a 1-bit
+    table, bit j of the pattern = parity of axis j); returned as a host int8 array."""
+    import ctypes as C
+    from grassmanntn_amd import _lib as L
+    from grassmanntn_amd._device import DeviceArray
+    n = len(chars)
+    table = [0] * 32
+    for m in range(1 << n):
+        par = {c: 0 for c in plan.chars}
+        for j, c in enumerate(chars):
+            par[c] = (m >> j) & 1
+        s = plan.s1(par) if which == 1 else plan.s3(par)
+        if s < 0:
+            table[m >> 5] |= 1 << (m & 31)
+    out = DeviceArray.empty(tuple(shape), np.int8)
+    dims = (C.c_int64 * n)(*shape)
+    tab = (C.c_uint32 * 32)(*table)
+    L.check(L.lib().gtn_sign_tensor(n, dims, tab, L.ptr(out), L.stream_ptr()), "gtn_sign_tensor")
+    return out.numpy()
+
+
 @pytest.fixture
 def rng():
     return np.random.default_rng(1234)

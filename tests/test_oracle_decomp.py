@@ -78,6 +78,43 @@ def test_trg_error_identity_at_full_bond(rng):
     assert err < 1e-12 and nrm > 0
 
 
+@pytest.mark.parametrize("direction", ["y", "x"])
+def test_atrg_error_identity_at_full_bond(direction):
+    """At bonds that truncate nothing, the ATRG step keeps the two-site partition function:
+    Z(T') = einsum('IJIK,iKiJ', T1, T2) (the reference's error_test, gauge2d.py:1855-1857); atrg2dx
+    is atrg2dy between the x/y leg swaps (gauge2d.py:1871-1885).  Every one of the step's eleven
+    contractions and three decompositions enters this identity."""
+    f = O.atrg2dy if direction == "y" else O.atrg2dx
+    T1, T2 = O.site_tensor(4, seed=5), O.site_tensor(4, seed=7)
+    for mode in ("sector", "literal"):
+        T, nrm, err = f(T1, T2, dcut=256, error_test=True, mode=mode)
+        assert err < 1e-12 and nrm > 0, (mode, err)
+    assert T.statistics == (1, 1, -1, -1)
+    assert T.shape == ((16, 4, 16, 4) if direction == "y" else (4, 16, 4, 16))
+
+
+def test_atrg_identity_detects_mistakes(monkeypatch):
+    """The full-bond identity is not satisfied by a step with a dropped square root (U S and S V
+    instead of U sqrt(S), sqrt(S) V: the reference's block-power defect) or with one sign of the
+    fermionic reordering flipped (an odd-odd exchange counted as even)."""
+    T1, T2 = O.site_tensor(4, seed=5), O.site_tensor(4, seed=7)
+    with monkeypatch.context() as m:
+        m.setattr(O, "sqrt", lambda S: S)
+        _, _, err = O.atrg2dy(T1, T2, dcut=256, error_test=True)
+        assert err > 1e-3
+    real_perm_sign = O.perm_sign
+
+    def wrong_perm_sign(src, dst, odd):
+        s = real_perm_sign(src, dst, odd)
+        n_odd = sum(1 for c in set(src) if odd.get(c, 0) % 2 == 1)
+        return -s if n_odd >= 2 else s
+
+    with monkeypatch.context() as m:
+        m.setattr(O, "perm_sign", wrong_perm_sign)
+        _, _, err = O.atrg2dy(T1, T2, dcut=256, error_test=True, mode="literal")
+        assert err > 1e-6
+
+
 def test_trg_literal_equals_sector():
     T = O.site_tensor(4, seed=6)
     a, na = O.trg(T, dcut=4, mode="literal")
