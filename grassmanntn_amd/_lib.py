@@ -98,6 +98,8 @@ def _bind(lib):
         "gtn_permute": (C.c_int, [C.POINTER(PermDesc), vp, vp, vp]),
         "gtn_sign_tensor": (C.c_int, [i32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), vp, vp]),
         "gtn_gemm_grouped": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),
+        "gtn_gemm_grouped_noalloc": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),
+        "gtn_gemm_noalloc_max_scratch": (C.c_int64, []),
         "gtn_mfma_selftest": (C.c_int, [vp, vp]),
         "gtn_sumsq": (C.c_int, [vp, i64, i32, vp, vp]),
         "gtn_scale_by_norm": (C.c_int, [vp, i64, i32, vp, vp]),

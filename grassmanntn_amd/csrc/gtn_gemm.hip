@@ -472,8 +472,41 @@ static int split_target() {
 
 using namespace gtn;
 
-extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream) {
-    GTN_REQUIRE(n_groups >= 0 && (n_groups == 0 || groups), "gtn_gemm_grouped: bad groups");
+// Split-reduction scratch of gtn_gemm_grouped_noalloc: one fixed array per device, held for the life
+// of the process.  In host code `scratch` only names the symbol; its device address on the current
+// device comes from hipGetSymbolAddress (scratch_ptr below) and is what kernels are given.
+constexpr int64_t kNoAllocScratchBytes = 64ll << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// device address of `scratch` on the current device, looked up once per device
+static int scratch_ptr(void **out) {
+    int dev;
+    GTN_CHECK_HIP(hipGetDevice(&dev));
+    GTN_REQUIRE(dev >= 0 && dev < kMaxDevices, "gtn_gemm_grouped_noalloc: device index %d out of range", dev);
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("gtn_gemm_grouped_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return GTN_EDEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GTN_OK;
+}
+
+// The grouped GEMM.  fixed_ws == nullptr: split-reduction partials go to the library's per-device
+// workspace (allocated and grown here).  Otherwise they go to fixed_ws (fixed_bytes long) and a
+// split plan that needs more is an error; nothing is allocated.
+static int gemm_grouped_impl(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream,
+                             void *fixed_ws, int64_t fixed_bytes, const char *name) {
+    GTN_REQUIRE(n_groups >= 0 && (n_groups == 0 || groups), "%s: bad groups", name);
     GTN_REQUIRE(dtype == GTN_F64 || dtype == GTN_F32, "gtn_gemm_grouped: dtype must be F64 or F32");
     const int tile = pick_tile(groups, n_groups);
     const int BM = tile, BN = tile;
@@ -571,12 +604,23 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
                 }
             }
             if (splitting) {
-                int dev;
-                GTN_CHECK_HIP(hipGetDevice(&dev));
-                lk.lock();
-                ws = &g_ws[dev];
-                int rc = split_workspace(*ws, (size_t)need * es, s);
-                if (rc) return rc;
+                char *base_ws;
+                if (fixed_ws) {
+                    if (need * es > fixed_bytes) {
+                        set_error("%s: the split reduction needs %lld bytes of scratch, above its maximum of %lld bytes",
+                                  name, (long long)(need * es), (long long)fixed_bytes);
+                        return GTN_ENOMEM;
+                    }
+                    base_ws = (char *)fixed_ws;
+                } else {
+                    int dev;
+                    GTN_CHECK_HIP(hipGetDevice(&dev));
+                    lk.lock();
+                    ws = &g_ws[dev];
+                    int rc = split_workspace(*ws, (size_t)need * es, s);
+                    if (rc) return rc;
+                    base_ws = (char *)ws->p;
+                }
                 orig.assign(a.g, a.g + used);
                 int64_t off = 0;
                 tiles = 0;
@@ -584,7 +628,7 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
                     GemmGroup &g = a.g[i];
                     a.tile_start[i] = tiles;
                     if (g.splits > 1) {
-                        g.C = (char *)ws->p + off * es;
+                        g.C = base_ws + off * es;
                         g.scm = g.N;
                         g.scn = 1;
                         g.scs = (int64_t)g.M * g.N;
@@ -600,7 +644,7 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
         }
         int rc = dtype == GTN_F64 ? launch_gemm<double>(a, precision, tile, s) : launch_gemm<float>(a, precision, tile, s);
         if (rc) return rc;
-        if (ws) {
+        if (!orig.empty()) {
             for (int i = 0; i < used; ++i) {
                 const GemmGroup &g = a.g[i], &o = orig[i];
                 if (g.splits == 1) continue;
@@ -613,12 +657,29 @@ extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int d
                                        g.batch, (float *)o.C, o.scm, o.scn, o.scb, o.alpha, o.beta);
                 GTN_CHECK_LAUNCH();
             }
-            GTN_CHECK_HIP(hipEventRecord(ws->done, s));
-            ws->last = s;
-            ws->used = true;
+            if (ws) {
+                GTN_CHECK_HIP(hipEventRecord(ws->done, s));
+                ws->last = s;
+                ws->used = true;
+            }
         }
     }
     return GTN_OK;
+}
+
+extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream) {
+    return gemm_grouped_impl(groups, n_groups, dtype, precision, stream, nullptr, 0, "gtn_gemm_grouped");
+}
+
+extern "C" int64_t gtn_gemm_noalloc_max_scratch(void) { return kNoAllocScratchBytes; }
+
+extern "C" int gtn_gemm_grouped_noalloc(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision,
+                                        void *stream) {
+    void *p = nullptr;
+    int rc = scratch_ptr(&p);
+    if (rc) return rc;
+    return gemm_grouped_impl(groups, n_groups, dtype, precision, stream, p, kNoAllocScratchBytes,
+                             "gtn_gemm_grouped_noalloc");
 }
 
 extern "C" int gtn_mfma_selftest(double *out_dev, void *stream) {

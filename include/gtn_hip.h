@@ -158,6 +158,20 @@ typedef struct gtn_gemm_desc {
 } gtn_gemm_desc;
 int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream);
 
+/* Allocation-free variant of gtn_gemm_grouped.  It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels, split plan and results (bit for bit)
+ * as gtn_gemm_grouped; the split-reduction partials that gtn_gemm_grouped keeps in a workspace it
+ * allocates go instead to a fixed per-device scratch array of gtn_gemm_noalloc_max_scratch() bytes
+ * (64 MiB) that the library holds from its first use on a device until the process ends.  Nothing
+ * is allocated or freed here.  A call whose split plan needs more scratch than that fails with
+ * GTN_ENOMEM and a message giving both sizes.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap in time on one GPU (different streams, different host threads) race; calls
+ * ordered on one stream do not.  Nothing in the scratch needs to start at zero (every partial is
+ * written before it is read). */
+int gtn_gemm_grouped_noalloc(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream);
+int64_t gtn_gemm_noalloc_max_scratch(void);
+
 /* Device self-test of the matrix-core operand/result lane layouts the GEMM kernels assume (no
  * counterpart in the reference: a diagnostic for users of this library).  One wavefront runs each
  * MFMA form (FP64 16x16x4, FP32 16x16x4, BF16 16x16x32) on inputs whose product is

@@ -28,6 +28,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
 #define __shared__ static
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_lds();
 
@@ -210,6 +211,12 @@ hipError_t hipEventDestroy(hipEvent_t e);
 #define hipEventDisableTiming 2
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
+// a __device__ variable is a host variable here: its address is the symbol's address
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+    *p = const_cast<void *>(sym);
+    return hipSuccess;
+}
 // device-side integer min / max of HIP
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
