@@ -109,6 +109,16 @@ def _bind(lib):
         "gtn_elementwise": (C.c_int, [vp, i64, i32, i32, d, d, vp]),
         "gtn_axpby": (C.c_int, [vp, vp, vp, i64, i32, d, d, d, d, vp]),
         "gtn_odd_norm": (C.c_int, [vp, i32, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32), i32, vp, vp]),
+        "gtn_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+        "gtn_stream_destroy": (C.c_int, [vp]),
+        "gtn_stream_synchronize": (C.c_int, [vp]),
+        "gtn_graph_begin": (C.c_int, [vp, i32]),
+        "gtn_graph_end": (C.c_int, [vp, C.POINTER(C.c_void_p)]),
+        "gtn_graph_launch": (C.c_int, [vp, vp]),
+        "gtn_graph_destroy": (C.c_int, [vp]),
+        "gtn_pool_release": (C.c_int, [i32]),
+        "gtn_normalize_batch": (C.c_int, [vp, i64, i64, i32, vp, vp, vp, i32, vp, vp]),
+        "gtn_diag_interleave": (C.c_int, [vp, i64, i64, vp, i32, vp]),
         "gtn_svd_workspace_bytes": (C.c_int64, [C.POINTER(SvdBlock), i32, C.POINTER(SvdOpts)]),
         "gtn_svd_blocks": (C.c_int, [C.POINTER(SvdBlock), i32, C.POINTER(SvdOpts), vp, i64, vp]),
     }
@@ -150,9 +160,18 @@ def check(rc, what=""):
         raise GrassmannDeviceError("%s failed (code %d): %s" % (what, rc, msg))
 
 
+_stream = None
+
+
 def stream_ptr():
-    """All host-layer work is ordered on the null stream."""
-    return None
+    """The stream all host-layer work is ordered on: a non-blocking stream the library creates on
+    first use (graph capture needs a stream other than the null stream)."""
+    global _stream
+    if _stream is None:
+        s = C.c_void_p()
+        check(lib().gtn_stream_create(C.byref(s)), "gtn_stream_create")
+        _stream = s
+    return _stream
 
 
 def ptr(t):

@@ -48,6 +48,12 @@ int check_range(const void *p, int64_t bytes, const char *what);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// graph bookkeeping of the runtime (gtn_runtime.hip): instantiated graphs that may still be
+// launched, and whether a capture is open.  Library-owned device buffers that a graph may hold are
+// never freed or moved while either is nonzero.
+int graphs_alive();
+int capture_open();
+
 inline int grid_for(int64_t n, int block, int max_blocks = 65536) {
     int64_t g = (n + block - 1) / block;
     if (g > max_blocks) g = max_blocks;

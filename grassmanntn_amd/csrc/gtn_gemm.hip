@@ -382,12 +382,18 @@ struct SplitWs {
 static std::mutex g_ws_mu;
 static std::map<int, SplitWs> g_ws;
 
+static std::vector<void *> g_ws_retired;  // outgrown workspaces a graph may still hold
+
 static int split_workspace(SplitWs &w, size_t bytes, hipStream_t s) {
     if (w.used && w.last != s) GTN_CHECK_HIP(hipStreamWaitEvent(s, w.done, 0));
     if (w.bytes < bytes) {
         if (w.p) {
-            GTN_CHECK_HIP(hipDeviceSynchronize());
-            GTN_CHECK_HIP(hipFree(w.p));
+            if (graphs_alive() || capture_open()) {
+                g_ws_retired.push_back(w.p);  // kept: a captured graph may address it
+            } else {
+                GTN_CHECK_HIP(hipDeviceSynchronize());
+                GTN_CHECK_HIP(hipFree(w.p));
+            }
             w.p = nullptr;
             w.bytes = 0;
         }

@@ -523,6 +523,59 @@ __global__ void k_lognorm(double *ln, const double *nrm2) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ln[0] = 2.0 * ln[0] + 0.5 * log(nrm2[0]);
 }
 
+// ---- batched normalisation: every point b of a batch of nb tensors (n elements each) --------------
+// pass 1: per-point partial sums of squares, grid (G, nb); part[b * G + g]
+template <typename T>
+__global__ void k_sumsq_batch_partial(const T *__restrict__ x, int64_t n, double *part) {
+    __shared__ double sh[16];
+    const T *xb = x + (int64_t)blockIdx.y * n;
+    double acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += sq(xb[i]);
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+// pass 2, one workgroup per point: nrm2[b]; the driver's log-norm rule (example.py:192)
+// lognorm[b] = 2 lognorm[b] + log ||T_b||; hist[b * hist_len + step] = log ||T_b|| while step < hist_len
+__global__ void k_norm_batch_final(const double *part, int G, double *nrm2, double *lognorm, double *hist,
+                                   int hist_len, const double *step) {
+    __shared__ double sh[16];
+    const int b = blockIdx.x;
+    double acc = 0;
+    for (int i = threadIdx.x; i < G; i += blockDim.x) acc += part[(int64_t)b * G + i];
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        nrm2[b] = t;
+        const double ln = 0.5 * log(t);
+        if (lognorm) lognorm[b] = 2.0 * lognorm[b] + ln;
+        if (hist && step) {
+            const int k = (int)step[0];
+            if (k >= 0 && k < hist_len) hist[(int64_t)b * hist_len + k] = ln;
+        }
+    }
+}
+// pass 3: x_b /= ||x_b||, grid (G, nb)
+template <typename T>
+__global__ void k_scale_batch(T *x, int64_t n, const double *nrm2) {
+    const double inv = 1.0 / sqrt(nrm2[blockIdx.y]);
+    T *xb = x + (int64_t)blockIdx.y * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        xb[i] = (T)((double)xb[i] * inv);
+}
+__global__ void k_step_increment(double *step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) step[0] += 1.0;
+}
+// dst (nb, 2d, 2d) = diagonal matrices whose entry (r, r) is src[b][r % 2][r / 2] (src (nb, 2, d)):
+// the singular values of the even and odd parity blocks interleaved by the parity-preserving index
+template <typename T>
+__global__ void k_diag_interleave(T *dst, int64_t nb, int64_t d, const T *src) {
+    const int64_t w = 2 * d, total = nb * w * w;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = e / (w * w), rc = e - b * w * w, r = rc / w, c = rc - r * w;
+        dst[e] = (r == c) ? src[b * w + (r & 1) * d + (r >> 1)] : (T)0;
+    }
+}
+
 template <typename T>
 __global__ void k_power_diag(T *x, int64_t n, int64_t stride, double p, double rcond) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -584,16 +637,17 @@ __global__ void k_axpby_c(const double2 *x, const double2 *y, double2 *out, int6
     }
 }
 
-static double *g_partials = nullptr;  // 1024 doubles per device, grown on demand
-static int g_partials_dev = -1;
-static int partial_buffer(double **p) {
+// Partial sums of the two-pass reductions: 1024 doubles per (device, stream), so that reductions
+// queued on different streams never share a buffer.  Kept for the life of the process.
+static std::mutex g_part_mu;
+static std::map<std::pair<int, hipStream_t>, double *> g_partials;
+static int partial_buffer(hipStream_t s, double **p) {
     int dev;
     GTN_CHECK_HIP(hipGetDevice(&dev));
-    if (!g_partials || g_partials_dev != dev) {
-        GTN_CHECK_HIP(hipMalloc(&g_partials, 1024 * sizeof(double)));
-        g_partials_dev = dev;
-    }
-    *p = g_partials;
+    std::lock_guard<std::mutex> l(g_part_mu);
+    double *&b = g_partials[std::make_pair(dev, s)];
+    if (!b) GTN_CHECK_HIP(hipMalloc(&b, 1024 * sizeof(double)));
+    *p = b;
     return GTN_OK;
 }
 
@@ -823,7 +877,9 @@ static int tab_tables(const PermArgs &a, const TabBuild &tb, hipStream_t s, TabA
     auto it = g_tab_cache.find(key);
     if (it == g_tab_cache.end()) {
         const int64_t n = tb.start[tb.ng];
-        if (g_tab_bytes + n * (int64_t)sizeof(TabEntry) > kTabCacheBytes && !g_tab_cache.empty()) {
+        // over budget: drop the cache, unless a graph may hold some of its tables
+        if (g_tab_bytes + n * (int64_t)sizeof(TabEntry) > kTabCacheBytes && !g_tab_cache.empty() &&
+            !graphs_alive() && !capture_open()) {
             GTN_CHECK_HIP(hipDeviceSynchronize());
             for (auto &e : g_tab_cache) GTN_CHECK_HIP(hipFree(e.second.dev));
             g_tab_cache.clear();
@@ -831,10 +887,17 @@ static int tab_tables(const PermArgs &a, const TabBuild &tb, hipStream_t s, TabA
         }
         TabEntry *p = nullptr;
         GTN_CHECK_HIP(hipMalloc(&p, n * sizeof(TabEntry)));
-        hipLaunchKernelGGL(k_perm_tables, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, tb, p);
-        GTN_CHECK_LAUNCH();
-        // the tables are shared by later gathers on any stream: finish them here
-        GTN_CHECK_HIP(hipStreamSynchronize(s));
+        // built on a stream of their own and finished here: the tables are shared by later gathers
+        // on any stream, and the caller's stream may be capturing a graph
+        hipStream_t bs = nullptr;
+        GTN_CHECK_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+        hipLaunchKernelGGL(k_perm_tables, dim3(grid_for(n, 256, 4096)), dim3(256), 0, bs, a, tb, p);
+        hipError_t le = hipGetLastError();
+        hipError_t se = hipStreamSynchronize(bs);
+        (void)hipStreamDestroy(bs);
+        GTN_CHECK_HIP(le);
+        GTN_CHECK_HIP(se);
+        (void)s;
         it = g_tab_cache.emplace(key, TabCacheEntry{p, n}).first;
         g_tab_bytes += n * (int64_t)sizeof(TabEntry);
     }
@@ -931,11 +994,11 @@ int gtn_sign_tensor(int ndim, const int64_t *dims, const uint32_t *table, int8_t
 }
 
 int gtn_sumsq(const void *x, int64_t n, int dtype, double *out, void *stream) {
+    hipStream_t s = as_stream(stream);
     double *part;
-    int rc = partial_buffer(&part);
+    int rc = partial_buffer(s, &part);
     if (rc) return rc;
     int g = grid_for(n, 256, 1024);
-    hipStream_t s = as_stream(stream);
     switch (dtype) {
         case GTN_F64: hipLaunchKernelGGL(k_sumsq_partial<double>, dim3(g), dim3(256), 0, s, (const double *)x, n, part); break;
         case GTN_F32: hipLaunchKernelGGL(k_sumsq_partial<float>, dim3(g), dim3(256), 0, s, (const float *)x, n, part); break;
@@ -1024,6 +1087,53 @@ int gtn_axpby(const void *x, const void *y, void *out, int64_t n, int dtype, dou
     return GTN_OK;
 }
 
+int gtn_normalize_batch(void *x, int64_t nbatch, int64_t n, int dtype, double *nrm2, double *lognorm, double *hist,
+                        int hist_len, double *step, void *stream) {
+    GTN_REQUIRE(nbatch >= 1 && nbatch <= 1024 && n >= 1, "gtn_normalize_batch: need 1 <= nbatch <= 1024 and n >= 1");
+    GTN_REQUIRE(x && nrm2, "gtn_normalize_batch: null buffer");
+    GTN_REQUIRE(dtype == GTN_F64 || dtype == GTN_F32, "gtn_normalize_batch: dtype must be F64 or F32");
+    hipStream_t s = as_stream(stream);
+    double *part;
+    int rc = partial_buffer(s, &part);
+    if (rc) return rc;
+    const int G = (int)(1024 / nbatch) < grid_for(n, 256, 1024) ? (int)(1024 / nbatch) : grid_for(n, 256, 1024);
+    const dim3 g2((unsigned)G, (unsigned)nbatch), b(256);
+    if (dtype == GTN_F64)
+        hipLaunchKernelGGL(k_sumsq_batch_partial<double>, g2, b, 0, s, (const double *)x, n, part);
+    else
+        hipLaunchKernelGGL(k_sumsq_batch_partial<float>, g2, b, 0, s, (const float *)x, n, part);
+    GTN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_norm_batch_final, dim3((unsigned)nbatch), dim3(256), 0, s, (const double *)part, G, nrm2,
+                       lognorm, hist, hist_len, (const double *)step);
+    GTN_CHECK_LAUNCH();
+    const dim3 g3((unsigned)grid_for(n, 256, (int)(65536 / nbatch) > 0 ? (int)(65536 / nbatch) : 1), (unsigned)nbatch);
+    if (dtype == GTN_F64)
+        hipLaunchKernelGGL(k_scale_batch<double>, g3, b, 0, s, (double *)x, n, (const double *)nrm2);
+    else
+        hipLaunchKernelGGL(k_scale_batch<float>, g3, b, 0, s, (float *)x, n, (const double *)nrm2);
+    GTN_CHECK_LAUNCH();
+    if (step) {
+        hipLaunchKernelGGL(k_step_increment, dim3(1), dim3(64), 0, s, step);
+        GTN_CHECK_LAUNCH();
+    }
+    return GTN_OK;
+}
+
+int gtn_diag_interleave(void *dst, int64_t nbatch, int64_t d, const void *src, int dtype, void *stream) {
+    GTN_REQUIRE(nbatch >= 0 && d >= 0 && (nbatch * d == 0 || (dst && src)), "gtn_diag_interleave: bad argument");
+    if (nbatch * d == 0) return GTN_OK;
+    hipStream_t s = as_stream(stream);
+    const int64_t total = nbatch * 4 * d * d;
+    dim3 g(grid_for(total, 256, 1 << 16)), b(256);
+    switch (dtype) {
+        case GTN_F64: hipLaunchKernelGGL(k_diag_interleave<double>, g, b, 0, s, (double *)dst, nbatch, d, (const double *)src); break;
+        case GTN_F32: hipLaunchKernelGGL(k_diag_interleave<float>, g, b, 0, s, (float *)dst, nbatch, d, (const float *)src); break;
+        default: set_error("gtn_diag_interleave: dtype must be F64 or F32"); return GTN_ENOTSUP;
+    }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
 int gtn_odd_norm(const void *x, int ndim, const int64_t *dims, const int32_t *ferm, const int32_t *pp, int dtype,
                  double *out, void *stream) {
     GTN_REQUIRE(ndim >= 0 && ndim <= GTN_PERM_MAX_AX, "gtn_odd_norm: ndim out of range");
@@ -1037,11 +1147,11 @@ int gtn_odd_norm(const void *x, int ndim, const int64_t *dims, const int32_t *fe
         if (ferm[i]) a.ferm_mask |= 1 << i;
         if (pp[i]) a.pp_mask |= 1 << i;
     }
+    hipStream_t s = as_stream(stream);
     double *part;
-    int rc = partial_buffer(&part);
+    int rc = partial_buffer(s, &part);
     if (rc) return rc;
     int g = grid_for(n, 256, 1024);
-    hipStream_t s = as_stream(stream);
     switch (dtype) {
         case GTN_F64: hipLaunchKernelGGL(k_odd_partial<double>, dim3(g), dim3(256), 0, s, a, (const double *)x, n, part); break;
         case GTN_F32: hipLaunchKernelGGL(k_odd_partial<float>, dim3(g), dim3(256), 0, s, a, (const float *)x, n, part); break;

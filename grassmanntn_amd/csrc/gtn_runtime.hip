@@ -1,10 +1,19 @@
-// Device memory, copies, fills, events and synchronisation for the Python host layer.
+// Device memory, copies, fills, events, streams, HIP graphs and synchronisation for the Python host
+// layer.
 //
 // The library owns its HBM through a small caching allocator (exact size classes rounded to
 // 256 bytes, reused in stream order on the one stream the host layer uses), so that processes using
 // grassmanntn_amd need no other GPU framework and run on the HIP runtime this library was built
 // against.
+//
+// Graph capture and memory pools: while a capture is open (gtn_graph_begin .. gtn_graph_end),
+// allocations come from the capture's pool.  A pool block freed at any time returns to that pool
+// only, so the addresses a captured graph holds are never handed to work outside it; blocks freed
+// inside the capture are reused by later allocations of the same capture in stream order, which is
+// also the order of every replay.  gtn_pool_release hands a pool's blocks back to the general cache
+// once its graphs are destroyed.
 #include "gtn_common.h"
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string.h>
@@ -14,9 +23,26 @@ namespace gtn {
 
 static std::mutex g_mu;
 static std::map<int64_t, std::vector<void *>> g_free;  // size class -> cached blocks
-static std::map<void *, int64_t> g_live;               // pointer -> size class
+struct LiveBlock {
+    int64_t cls;
+    int pool;
+};
+static std::map<void *, LiveBlock> g_live;             // pointer -> size class, pool (0: general)
 static int64_t g_cached_bytes = 0;
 static const int64_t kCacheCap = 64ll << 30;          // release cached blocks beyond 64 GiB
+struct Pool {
+    std::map<int64_t, std::vector<void *>> free;
+    bool released = false;
+};
+static std::map<int, Pool> g_pools;  // pool id (> 0) -> its free blocks
+static int g_cur_pool = 0;           // pool of the open capture, 0 outside captures
+static std::atomic<int> g_graphs{0};
+
+int graphs_alive() { return g_graphs.load(); }
+int capture_open() {
+    std::lock_guard<std::mutex> l(g_mu);
+    return g_cur_pool != 0;
+}
 
 template <typename T>
 __global__ void k_fill_value(T *p, int64_t n, T v) {
@@ -43,20 +69,23 @@ int gtn_malloc(int64_t bytes, void **out) {
     GTN_REQUIRE(out && bytes >= 0, "gtn_malloc: bad argument");
     int64_t cls = (bytes + 255) / 256 * 256;
     if (cls == 0) cls = 256;
+    int pool;
     {
         std::lock_guard<std::mutex> l(g_mu);
-        auto it = g_free.find(cls);
-        if (it != g_free.end() && !it->second.empty()) {
+        pool = g_cur_pool;
+        auto &fl = pool ? g_pools[pool].free : g_free;
+        auto it = fl.find(cls);
+        if (it != fl.end() && !it->second.empty()) {
             *out = it->second.back();
             it->second.pop_back();
-            g_cached_bytes -= cls;
-            g_live[*out] = cls;
+            if (!pool) g_cached_bytes -= cls;
+            g_live[*out] = LiveBlock{cls, pool};
             return GTN_OK;
         }
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, (size_t)cls);
-    if (e != hipSuccess) {
+    if (e != hipSuccess && !pool) {
         // give the cache back to the device and retry once
         std::lock_guard<std::mutex> l(g_mu);
         for (auto &kv : g_free)
@@ -70,7 +99,7 @@ int gtn_malloc(int64_t bytes, void **out) {
         return GTN_ENOMEM;
     }
     std::lock_guard<std::mutex> l(g_mu);
-    g_live[p] = cls;
+    g_live[p] = LiveBlock{cls, pool};
     *out = p;
     return GTN_OK;
 }
@@ -80,9 +109,18 @@ int gtn_free(void *p) {
     std::lock_guard<std::mutex> l(g_mu);
     auto it = g_live.find(p);
     GTN_REQUIRE(it != g_live.end(), "gtn_free: pointer not allocated by gtn_malloc");
-    int64_t cls = it->second;
+    int64_t cls = it->second.cls;
+    int pool = it->second.pool;
     g_live.erase(it);
-    if (g_cached_bytes + cls > kCacheCap) {
+    if (pool) {
+        auto pit = g_pools.find(pool);
+        if (pit != g_pools.end() && !pit->second.released) {
+            pit->second.free[cls].push_back(p);
+            return GTN_OK;
+        }
+    }
+    // the general cache; never hipFree while a capture is open or a graph may hold the block
+    if (g_cached_bytes + cls > kCacheCap && g_cur_pool == 0 && g_graphs.load() == 0) {
         GTN_CHECK_HIP(hipFree(p));
         return GTN_OK;
     }
@@ -150,6 +188,100 @@ int gtn_copy2d(void *dst, int64_t ldd, const void *src, int64_t lds, int64_t row
 
 int gtn_synchronize(void) {
     GTN_CHECK_HIP(hipDeviceSynchronize());
+    return GTN_OK;
+}
+
+int gtn_stream_create(void **stream) {
+    GTN_REQUIRE(stream, "gtn_stream_create: null argument");
+    hipStream_t s;
+    GTN_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void *)s;
+    return GTN_OK;
+}
+
+int gtn_stream_destroy(void *stream) {
+    if (stream) GTN_CHECK_HIP(hipStreamDestroy(as_stream(stream)));
+    return GTN_OK;
+}
+
+int gtn_stream_synchronize(void *stream) {
+    GTN_CHECK_HIP(hipStreamSynchronize(as_stream(stream)));
+    return GTN_OK;
+}
+
+int gtn_graph_begin(void *stream, int pool) {
+    GTN_REQUIRE(stream, "gtn_graph_begin: capture needs a created stream, not the null stream");
+    GTN_REQUIRE(pool > 0, "gtn_graph_begin: pool ids are positive");
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        GTN_REQUIRE(g_cur_pool == 0, "gtn_graph_begin: a capture is already open");
+        GTN_REQUIRE(!g_pools[pool].released, "gtn_graph_begin: pool %d was released", pool);
+        g_cur_pool = pool;
+    }
+    hipError_t e = hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> l(g_mu);
+        g_cur_pool = 0;
+        set_error("gtn_graph_begin: hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+        return GTN_EDEVICE;
+    }
+    return GTN_OK;
+}
+
+int gtn_graph_end(void *stream, void **graph_exec) {
+    GTN_REQUIRE(stream, "gtn_graph_end: null stream");
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        g_cur_pool = 0;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(as_stream(stream), &g);
+    if (e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        set_error("gtn_graph_end: hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        return GTN_EDEVICE;
+    }
+    if (!graph_exec) {  // capture abandoned
+        (void)hipGraphDestroy(g);
+        return GTN_OK;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_error("gtn_graph_end: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        return GTN_EDEVICE;
+    }
+    ++g_graphs;
+    *graph_exec = (void *)x;
+    return GTN_OK;
+}
+
+int gtn_graph_launch(void *graph_exec, void *stream) {
+    GTN_REQUIRE(graph_exec && stream, "gtn_graph_launch: null argument");
+    GTN_CHECK_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, as_stream(stream)));
+    return GTN_OK;
+}
+
+int gtn_graph_destroy(void *graph_exec) {
+    if (!graph_exec) return GTN_OK;
+    GTN_CHECK_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    --g_graphs;
+    return GTN_OK;
+}
+
+int gtn_pool_release(int pool) {
+    GTN_REQUIRE(pool > 0, "gtn_pool_release: pool ids are positive");
+    std::lock_guard<std::mutex> l(g_mu);
+    GTN_REQUIRE(g_cur_pool != pool, "gtn_pool_release: pool %d is in use by an open capture", pool);
+    Pool &P = g_pools[pool];
+    for (auto &kv : P.free)
+        for (void *q : kv.second) {
+            g_free[kv.first].push_back(q);
+            g_cached_bytes += kv.first;
+        }
+    P.free.clear();
+    P.released = true;
     return GTN_OK;
 }
 

@@ -75,6 +75,33 @@ int gtn_event_elapsed_ms(void *start, void *stop, float *ms);
 int gtn_event_destroy(void *ev);
 
 /* ------------------------------------------------------------------------------------------------
+ * Streams and HIP graphs (no counterpart in the reference, whose coarse-graining loop,
+ * example.py:162-196, is plain Python): the host layer orders its work on a stream created here, and
+ * a coarse-graining step issued between gtn_graph_begin and gtn_graph_end is recorded once and
+ * replayed with gtn_graph_launch.
+ * gtn_stream_create: a non-blocking stream at the default priority.
+ * gtn_graph_begin(stream, pool): work issued on `stream` is recorded (relaxed capture mode), not run,
+ *   until gtn_graph_end; meanwhile gtn_malloc serves allocations from memory pool `pool` (> 0).  A
+ *   pool's blocks, freed during or after the capture, are reused only by allocations from the same
+ *   pool, so every address a graph holds stays its own; inside one capture a freed block may be
+ *   reused in stream order, which every replay keeps.  One capture at a time per process.
+ * gtn_graph_end(stream, &exec): closes the capture and instantiates it (exec NULL: abandon it).
+ * gtn_graph_launch(exec, stream): one replay, ordered on `stream`.
+ * gtn_pool_release(pool): after the pool's graphs are destroyed, its blocks go back to the general
+ *   cache (blocks still referenced return there when freed).
+ * While a graph exists or a capture is open the library frees none of its own buffers (gather
+ * tables, split-K workspaces).
+ * ---------------------------------------------------------------------------------------------- */
+int gtn_stream_create(void **stream);
+int gtn_stream_destroy(void *stream);
+int gtn_stream_synchronize(void *stream);
+int gtn_graph_begin(void *stream, int pool);
+int gtn_graph_end(void *stream, void **graph_exec);
+int gtn_graph_launch(void *graph_exec, void *stream);
+int gtn_graph_destroy(void *graph_exec);
+int gtn_pool_release(int pool);
+
+/* ------------------------------------------------------------------------------------------------
  * (A1) index tables.  Replaces param.gparity / param.sgn / param.encoder (param.py:61-83; the
  * literal tables of param.py:4-6 are their values on [0, 65536)).
  * popc[i] = popcount(i), sigma[i] = +-1, enc[i] = eps(i), for 0 <= i < n.  Any pointer may be NULL.
@@ -210,6 +237,18 @@ int gtn_axpby(const void *x_dev, const void *y_dev, void *out_dev, int64_t n, in
               double alpha_im, double beta_re, double beta_im, void *stream);
 int gtn_odd_norm(const void *x_dev, int ndim, const int64_t *dims, const int32_t *fermionic,
                  const int32_t *pp, int dtype, double *out_dev, void *stream);
+/* Per-point normalisation of a batch of coarse-grained tensors with the driver's log-norm rule
+ * (gauge2d.py:1746-1748 and example.py:191-193 for every point at once): for b < nbatch (<= 1024),
+ * x_b = x[b n .. b n + n): nrm2[b] = ||x_b||^2 (FP64 accumulation), x_b /= ||x_b||,
+ * lognorm[b] = 2 lognorm[b] + log ||x_b|| (when lognorm is not NULL), hist[b hist_len + k] =
+ * log ||x_b|| with k = step[0] (when hist and step are given and k < hist_len), then step[0] += 1. */
+int gtn_normalize_batch(void *x_dev, int64_t nbatch, int64_t n, int dtype, double *nrm2_dev, double *lognorm_dev,
+                        double *hist_dev, int hist_len, double *step_dev, void *stream);
+/* The diagonal singular-value matrices of a batch of parity-blocked decompositions
+ * (get_full_matrix of BlockSVD, __init__.py:4017-4028): dst (nbatch, 2d, 2d) with
+ * dst[b][r][r] = src[b][r % 2][r / 2] (src (nbatch, 2, d): the even then the odd block's values) and
+ * zero off the diagonal. */
+int gtn_diag_interleave(void *dst_dev, int64_t nbatch, int64_t d, const void *src_dev, int dtype, void *stream);
 
 /* ------------------------------------------------------------------------------------------------
  * (A10/A11) truncated SVD / Hermitian eig of a batch of parity blocks (BlockSVD / SortedSVD,

@@ -211,6 +211,21 @@ hipError_t hipEventDestroy(hipEvent_t e);
 #define hipEventDisableTiming 2
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned flags);
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
+// streams run their work at once; graph capture is not emulated (hipErrorNotSupported)
+#define hipStreamNonBlocking 0x01
+typedef void *hipGraph_t;
+typedef void *hipGraphExec_t;
+typedef void *hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+#define hipErrorNotSupported 801
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
+hipError_t hipStreamDestroy(hipStream_t s);
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode m);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g);
+hipError_t hipGraphInstantiate(hipGraphExec_t *x, hipGraph_t g, hipGraphNode_t *n, char *log, size_t len);
+hipError_t hipGraphDestroy(hipGraph_t g);
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s);
+hipError_t hipGraphExecDestroy(hipGraphExec_t x);
 // a __device__ variable is a host variable here: its address is the symbol's address
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {

@@ -89,12 +89,33 @@ class FakeLib:
     def gtn_set_dry_run(self, n, bases, sizes):
         return 0
 
+    # memory pools as the runtime keeps them: a block allocated while a capture is open belongs to
+    # that capture's pool; freeing it returns it to the pool (it stays allocated, since a recorded
+    # graph may address it) for reuse by later allocations of the same pool
     def gtn_malloc(self, nbytes, out):
-        out._obj.value = self.reg.alloc(nbytes)
+        pool = getattr(self, "pool_open", 0)
+        if not pool:
+            out._obj.value = self.reg.alloc(nbytes)
+            return 0
+        cls = max((int(nbytes) + 255) // 256 * 256, 256)
+        self.pool_free = getattr(self, "pool_free", {})
+        self.pool_of = getattr(self, "pool_of", {})
+        fl = self.pool_free.setdefault(pool, {}).setdefault(cls, [])
+        addr = fl.pop() if fl else self.reg.alloc(cls)
+        self.pool_of[addr] = (pool, cls)
+        out._obj.value = addr
         return 0
 
     def gtn_free(self, p):
-        self.reg.free(p.value if isinstance(p, C.c_void_p) else p)
+        addr = p.value if isinstance(p, C.c_void_p) else p
+        pool_of = getattr(self, "pool_of", {})
+        if addr in pool_of:
+            pool, cls = pool_of[addr]
+            if pool not in getattr(self, "pool_released", set()):
+                self.pool_free[pool][cls].append(addr)
+                return 0
+            del pool_of[addr]
+        self.reg.free(addr)
         return 0
 
     def gtn_memcpy(self, dst, src, nbytes, kind, stream):
@@ -344,6 +365,90 @@ class FakeLib:
         self._arr(out, 1, np.float64, "odd out")[0] = float(np.sum(np.abs(v[(par % 2) == 1]) ** 2))
         return 0
 
+    def gtn_normalize_batch(self, x, nb, n, dtype, nrm2, lognorm, hist, hist_len, step, stream):
+        v = self._arr(x, nb * n, NP[dtype], "normalize x").reshape(nb, n)
+        q = (v.astype(np.float64) ** 2).sum(axis=1)
+        self._arr(nrm2, nb, np.float64, "normalize nrm2")[:] = q
+        ln = 0.5 * np.log(q)
+        if lognorm:
+            a = self._arr(lognorm, nb, np.float64, "normalize lognorm")
+            a[:] = 2 * a + ln
+        if hist and step:
+            k = int(self._arr(step, 1, np.float64, "normalize step")[0])
+            if 0 <= k < hist_len:
+                h = self._arr(hist, nb * hist_len, np.float64, "normalize hist").reshape(nb, hist_len)
+                h[:, k] = ln
+        v /= np.sqrt(q)[:, None].astype(v.dtype)
+        if step:
+            self._arr(step, 1, np.float64, "normalize step")[0] += 1
+        return 0
+
+    def gtn_diag_interleave(self, dst, nb, d, src, dtype, stream):
+        sv = self._arr(src, nb * 2 * d, NP[dtype], "diag src").reshape(nb, 2, d)
+        out = self._arr(dst, nb * 4 * d * d, NP[dtype], "diag dst").reshape(nb, 2 * d, 2 * d)
+        out[...] = 0
+        r = np.arange(2 * d)
+        out[:, r, r] = sv[:, r % 2, r // 2]
+        return 0
+
+    def gtn_gemm_grouped_noalloc(self, groups, n, dtype, prec, stream):
+        return self.gtn_gemm_grouped(groups, n, dtype, prec, stream)
+
+    def gtn_gemm_noalloc_max_scratch(self):
+        return 64 << 20
+
+    # ---- streams and graph capture: calls issued during a capture are recorded with copies of their
+    # ctypes arguments (the runtime copies kernel arguments when it records a launch) and replayed by
+    # gtn_graph_launch; allocations run at once; a synchronous host copy inside a capture is an error
+    def gtn_stream_create(self, p):
+        p._obj.value = 0x5157
+        return 0
+
+    def gtn_stream_destroy(self, s):
+        return 0
+
+    def gtn_stream_synchronize(self, s):
+        return 0
+
+    def gtn_graph_begin(self, stream, pool):
+        if getattr(self, "_rec", None) is not None:
+            return self._fail("gtn_graph_begin: a capture is already open")
+        self._rec = []
+        self.pool_open = pool
+        return 0
+
+    def gtn_graph_end(self, stream, out):
+        rec, self._rec = self._rec, None
+        self.pool_open = 0
+        if out is None:
+            return 0
+        self.graphs = getattr(self, "graphs", {})
+        gid = len(self.graphs) + 1
+        self.graphs[gid] = rec
+        out._obj.value = gid
+        return 0
+
+    def gtn_graph_launch(self, g, stream):
+        gid = g.value if isinstance(g, C.c_void_p) else g
+        for name, args in self.graphs[gid]:
+            rc = getattr(self, name)(*args)
+            if rc:
+                return rc
+        return 0
+
+    def gtn_graph_destroy(self, g):
+        gid = g.value if isinstance(g, C.c_void_p) else g
+        self.graphs.pop(gid, None)
+        return 0
+
+    def gtn_pool_release(self, pool):
+        self.pool_released = getattr(self, "pool_released", set()) | {pool}
+        for cls, blocks in getattr(self, "pool_free", {}).pop(pool, {}).items():
+            for addr in blocks:
+                del self.pool_of[addr]
+                self.reg.free(addr)
+        return 0
+
     def gtn_svd_workspace_bytes(self, blocks, n, opts):
         return 8
 
@@ -383,6 +488,46 @@ class FakeLib:
         return 0
 
 
+_NOT_RECORDED = {"gtn_malloc", "gtn_free", "gtn_version", "gtn_last_error", "gtn_device_info", "gtn_set_dry_run",
+                 "gtn_svd_workspace_bytes", "gtn_gemm_noalloc_max_scratch", "gtn_event_create", "gtn_event_destroy",
+                 "gtn_stream_create", "gtn_stream_destroy", "gtn_graph_begin", "gtn_graph_end", "gtn_graph_launch",
+                 "gtn_graph_destroy", "gtn_pool_release"}
+
+
+def _copy_arg(a):
+    if isinstance(a, (C.Structure, C.Array)):
+        return type(a).from_buffer_copy(a)
+    if type(a).__name__ == "CArgObject":
+        return C.byref(_copy_arg(a._obj))
+    if isinstance(a, C.c_void_p):
+        return C.c_void_p(a.value)
+    return a
+
+
+class _Recorder:
+    """The FakeLib seen through graph capture: while a capture is open, work calls are recorded."""
+
+    def __init__(self, fake):
+        self._fake = fake
+
+    def __getattr__(self, name):
+        f = getattr(self._fake, name)
+        if name in _NOT_RECORDED or not name.startswith("gtn_"):
+            return f
+        fake = self._fake
+
+        def call(*args):
+            if getattr(fake, "_rec", None) is None:
+                return f(*args)
+            if name in ("gtn_memcpy",) and args[3] in (1, 2):
+                return fake._fail("gtn_memcpy: a synchronous host copy inside a graph capture")
+            if name in ("gtn_synchronize", "gtn_stream_synchronize", "gtn_event_elapsed_ms"):
+                return fake._fail("%s inside a graph capture" % name)
+            fake._rec.append((name, tuple(_copy_arg(a) for a in args)))
+            return 0
+        return call
+
+
 @contextlib.contextmanager
 def emulate(real_lib_path=None):
     """Run grassmanntn_amd on the CPU (tests only), either against the numpy re-statement of the
@@ -393,12 +538,13 @@ def emulate(real_lib_path=None):
     if real_lib_path:
         fake = L._bind(C.CDLL(real_lib_path))
     else:
-        fake = FakeLib(reg)
-    saved = (L.lib, L.load_library, L._lib)
+        fake = _Recorder(FakeLib(reg))
+    saved = (L.lib, L.load_library, L._lib, L._stream)
     try:
         L.lib = lambda: fake
         L.load_library = lambda: fake
         L._lib = fake
+        L._stream = None
         yield fake
     finally:
-        L.lib, L.load_library, L._lib = saved
+        L.lib, L.load_library, L._lib, L._stream = saved
