@@ -34,9 +34,9 @@ def _mk(data, stats):
     return r
 
 
-def _blocks_batch(Mdata, P, m, n):
-    """(P, 2, m/2, n/2) contiguous copy of the parity blocks M[b][p::2, p::2] (BlockSVD's ME / MO,
-    __init__.py:3990-3996) of every point b."""
+def _blocks_batch(Mdata, P, m, n, out=None):
+    """(P, 2, m/2, n/2) contiguous copy (into `out` when given) of the parity blocks M[b][p::2, p::2]
+    (BlockSVD's ME / MO, __init__.py:3990-3996) of every point b."""
     mh, nh = m // 2, n // 2
     G = Perm()
     axes = ((P, m * n), (2, n + 1), (mh, 2 * n), (nh, 2))
@@ -45,7 +45,7 @@ def _blocks_batch(Mdata, P, m, n):
         v = G.add_virt(dim)
         G.add_comp(v, dim, [(i, 1)])
         G.add_out(dim, [(v, 1)])
-    return G.run(Mdata)
+    return G.run(Mdata, dst=out)
 
 
 def _interleave_batch(stack, P, rows, cols, row_stride, par_stride, batch_stride):
@@ -94,10 +94,25 @@ def svd_batch(calls, cutoff):
         if k < 1:
             raise ValueError("svd_batch: cutoff %s keeps no values" % cutoff)
         d = _bond(k) // 2
-        jobs.append(dict(T=T, left=left, right=right, z=z, P=P, M=M, m=m, n=n, mh=mh, nh=nh, k=k, d=d,
-                         blocks=_blocks_batch(M.data, P, m, n),
-                         SU=DeviceArray.zeros((P, 2, mh, d)), SV=DeviceArray.zeros((P, 2, d, nh)),
-                         sv=DeviceArray.zeros((P, 2, d))))
+        jobs.append(dict(T=T, left=left, right=right, z=z, P=P, M=M, m=m, n=n, mh=mh, nh=nh, k=k, d=d))
+    # calls of equal block shape share one buffer each for blocks, U, S and V, so that the device
+    # decomposition sees all their blocks at constant strides (one launch per stage)
+    shapes = {}
+    for j in jobs:
+        shapes.setdefault((j["mh"], j["nh"], j["d"]), []).append(j)
+    for (mh, nh, d), js in shapes.items():
+        tot = sum(j["P"] for j in js)
+        blocks = DeviceArray.empty((tot, 2, mh, nh))
+        SU, SV, sv = DeviceArray.zeros((tot, 2, mh, d)), DeviceArray.zeros((tot, 2, d, nh)), DeviceArray.zeros((tot, 2, d))
+        at = 0
+        for j in js:
+            P = j["P"]
+            j["blocks"] = blocks.flat(at * 2 * mh * nh, P * 2 * mh * nh, (P, 2, mh, nh))
+            _blocks_batch(j["M"].data, P, j["m"], j["n"], out=j["blocks"])
+            j["SU"] = SU.flat(at * 2 * mh * d, P * 2 * mh * d, (P, 2, mh, d))
+            j["SV"] = SV.flat(at * 2 * d * nh, P * 2 * d * nh, (P, 2, d, nh))
+            j["sv"] = sv.flat(at * 2 * d, P * 2 * d, (P, 2, d))
+            at += P
     nblk = sum(2 * j["P"] for j in jobs)
     arr = (L.SvdBlock * nblk)()
     t = 0
