@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgtn_hip.so")
 
 GTN_F64, GTN_F32, GTN_C128, GTN_I8 = 0, 1, 2, 3
-PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3}
+GTN_ACT_NONE, GTN_ACT_GELU = 0, 1
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16": 4, "bf16_split_a": 5}
 MAX_AX, MAX_COMP, MAX_BITS = 10, 16, 10
 
 
@@ -73,6 +74,13 @@ class SvdOpts(C.Structure):
     ]
 
 
+class AdamwOpts(C.Structure):
+    _fields_ = [
+        ("lr_max", C.c_double), ("lr_min", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+        ("eps", C.c_double), ("weight_decay", C.c_double), ("total_steps", C.c_int64),
+    ]
+
+
 _lib = None
 _checked = False
 
@@ -119,6 +127,12 @@ def _bind(lib):
         "gtn_pool_release": (C.c_int, [i32]),
         "gtn_normalize_batch": (C.c_int, [vp, i64, i64, i32, vp, vp, vp, i32, vp, vp]),
         "gtn_diag_interleave": (C.c_int, [vp, i64, i64, vp, i32, vp]),
+        "gtn_mlp_bias_act": (C.c_int, [vp, i64, i64, i64, vp, i32, vp, vp]),
+        "gtn_mlp_act_backward": (C.c_int, [vp, vp, i64, i32, vp]),
+        "gtn_mlp_colsum": (C.c_int, [vp, i64, i64, i64, vp, vp]),
+        "gtn_mlp_mse": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, vp, vp]),
+        "gtn_adamw": (C.c_int, [vp, vp, vp, vp, i64, C.POINTER(AdamwOpts), vp, vp]),
+        "gtn_ensemble_stats": (C.c_int, [vp, i64, i64, vp, vp, vp]),
         "gtn_svd_workspace_bytes": (C.c_int64, [C.POINTER(SvdBlock), i32, C.POINTER(SvdOpts)]),
         "gtn_svd_blocks": (C.c_int, [C.POINTER(SvdBlock), i32, C.POINTER(SvdOpts), vp, i64, vp]),
     }

@@ -136,6 +136,7 @@ def svd_batch(calls, cutoff):
     ws_bytes = lib.gtn_svd_workspace_bytes(arr, nblk, C.byref(opts))
     ws = DeviceArray.empty(max(ws_bytes // 8, 1), np.float64)
     L.check(lib.gtn_svd_blocks(arr, nblk, C.byref(opts), L.ptr(ws), ws_bytes, L.stream_ptr()), "gtn_svd_blocks")
+    svd_batch.last_values = [j["sv"] for j in jobs]
     out = []
     for j in jobs:
         P, m, n, mh, nh, d = j["P"], j["m"], j["n"], j["mh"], j["nh"], j["d"]
@@ -168,10 +169,12 @@ def svd_batch(calls, cutoff):
 
 def trg_batch_step(Tb, dcut):
     """The contractions of one TRG step (gauge2d.py:1660-1734) on a batch Tb[B, i, j, k, l]: returns
-    the unnormalised T'[B, i, j, k, l]."""
+    the unnormalised T'[B, i, j, k, l] and the singular values of the first decomposition (T1's
+    parity blocks, (B, 2, d): even then odd, each decreasing)."""
     T1 = core.permuted_view(Tb, "Bijkl->Bjkli")
     T2 = core.permuted_view(Tb, "Bijkl->Bklij")
     (U1, S1, V1), (U2, S2, V2) = svd_batch([(T1, "ab|cd"), (T2, "ab|cd")], dcut)
+    sv = svd_batch.last_values[0]
     r = core.sqrt(S1)
     U1 = einsum("Babx,Bxc->Babc", U1, r)
     V1 = einsum("Bax,Bxbc->Babc", r, V1)
@@ -180,7 +183,7 @@ def trg_batch_step(Tb, dcut):
     V2 = einsum("Bax,Bxbc->Babc", r, V2)
     VV = einsum("Bkwz,Blxw->Blxzk", V1, V2)
     UU = einsum("Byxi,Bzyj->Bjzxi", U1, U2)
-    return einsum("Blxzk,Bjzxi->Bijkl", VV, UU)
+    return einsum("Blxzk,Bjzxi->Bijkl", VV, UU), sv
 
 
 def normalize_batch(T, nrm2, lognorm=None, hist=None, step=None):
@@ -258,7 +261,9 @@ class BatchedTRG:
     and every step of a run is one graph launch; graphs=False runs the same steps eagerly.
 
     run(T0, steps) -> ln Z per site of every point after `steps` steps (complex where the final
-    supertrace is negative), with the per-step log norms in self.log_norms (P x steps)."""
+    supertrace is negative), with the per-step log norms in self.log_norms (P x steps) and the last
+    step's singular values of T1 (both parity blocks, decreasing, divided by the largest) in
+    self.final_sv (P x bond)."""
 
     def __init__(self, P, D, dcut, steps, schedule=None, graphs=True):
         self.P, self.D, self.dcut, self.steps = P, D, dcut, steps
@@ -271,22 +276,24 @@ class BatchedTRG:
         self.lognorm = DeviceArray.zeros(P)
         self.hist = DeviceArray.zeros((P, max(steps, 1)))
         self.stepc = DeviceArray.zeros(1)
+        self.svals = DeviceArray.zeros((P, self.Dn))  # singular values of the last step's T1 blocks
         self.pool = _new_pool() if graphs else 0
         self.graphs = {}
 
     # ---- one step: input buffer -> output buffer -------------------------------------------------
     def _step(self, x_in, x_out, prec, bookkeeping=True):
+        from ._device import copy2d
         T = wrap_batch(x_in, self.P)
         with precision(*prec):
-            Tn = trg_batch_step(T, self.dcut)
+            Tn, sv = trg_batch_step(T, self.dcut)
         if bookkeeping:
             normalize_batch(Tn, self.nrm2, self.lognorm, self.hist, self.stepc)
+            copy2d(self.svals, self.Dn, sv.reshape((self.P, self.Dn)), self.Dn, self.P, self.Dn)
         else:
             normalize_batch(Tn, DeviceArray.empty(self.P))
         src = Tn.data
         cols = 4096 if src.numel() % (4096 * self.P) == 0 else src.numel() // self.P
         rows = src.numel() // cols
-        from ._device import copy2d
         copy2d(x_out.reshape((rows, cols)), cols, src.reshape((rows, cols)), cols, rows, cols)
 
     def _key(self, i):
@@ -345,6 +352,8 @@ class BatchedTRG:
             lz = logz_batch(T)
         logn = self.lognorm.numpy()
         self.log_norms = self.hist.numpy()
+        sv = -np.sort(-self.svals.numpy(), axis=1)
+        self.final_sv = sv / np.where(sv[:, :1] > 0, sv[:, :1], 1.0)
         return (lz + logn) / 2.0 ** self.steps
 
     def run(self, T0):

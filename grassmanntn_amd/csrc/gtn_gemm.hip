@@ -211,8 +211,11 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
 // A[i = l%16][k = 8*(l/16) + t], t = 0..7; B values B[k = 8*(l/16) + t][j = l%16]; the result layout
 // is that of the 16x16x4 instructions.
 // ------------------------------------------------------------------------------------------------
-template <typename TIN, int NSPLIT, int BM, int BN, int BK>
+// SA / SB: BF16 terms each A / B element is split into (hi, mid, lo); the products of term t of A
+// and term u of B with t + u < max(SA, SB) are accumulated, smallest first.
+template <typename TIN, int SA, int SB, int BM, int BN, int BK>
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
+    constexpr int NSPLIT = SA > SB ? SA : SB;
     int gi, b, tm, tn;
     if (!locate(args, gi, b, tm, tn)) return;
     const GemmGroup &g = args.g[gi];
@@ -224,8 +227,8 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
     const TIN *B = (const TIN *)g.B + b * g.sbb + kb * g.sbk;
     TIN *C = (TIN *)g.C + b * g.scb + sp * g.scs;
     constexpr int PAD = 8;
-    __shared__ __attribute__((aligned(16))) __bf16 As[NSPLIT][BM][BK + PAD];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[NSPLIT][BN][BK + PAD];
+    __shared__ __attribute__((aligned(16))) __bf16 As[SA][BM][BK + PAD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[SB][BN][BK + PAD];
     constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
     constexpr int WM = BM / 2, WN = BN / 2, TI = WM / 16, TJ = WN / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -250,7 +253,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
             panel_coord<BM, BK>(s, akc, r, k);
             double x = ra[s];
 #pragma unroll
-            for (int t = 0; t < NSPLIT; ++t) {
+            for (int t = 0; t < SA; ++t) {
                 __bf16 h = (__bf16)(float)x;
                 As[t][r][k] = h;
                 x -= (double)(float)h;
@@ -262,7 +265,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
             panel_coord<BN, BK>(s, bkc, r, k);
             double x = rb[s];
 #pragma unroll
-            for (int t = 0; t < NSPLIT; ++t) {
+            for (int t = 0; t < SB; ++t) {
                 __bf16 h = (__bf16)(float)x;
                 Bs[t][r][k] = h;
                 x -= (double)(float)h;
@@ -275,29 +278,30 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs args) {
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 32) {
-            bf8 af[NSPLIT][TI], bfr[NSPLIT][TJ];
+            bf8 af[SA][TI], bfr[SB][TJ];
 #pragma unroll
-            for (int t = 0; t < NSPLIT; ++t) {
+            for (int t = 0; t < SA; ++t)
 #pragma unroll
                 for (int i = 0; i < TI; ++i)
                     af[t][i] = *(const bf8 *)&As[t][wm + 16 * i + lane % 16][kk + 8 * (lane / 16)];
 #pragma unroll
+            for (int t = 0; t < SB; ++t)
+#pragma unroll
                 for (int j = 0; j < TJ; ++j)
                     bfr[t][j] = *(const bf8 *)&Bs[t][wn + 16 * j + lane % 16][kk + 8 * (lane / 16)];
-            }
 #pragma unroll
             for (int i = 0; i < TI; ++i)
 #pragma unroll
                 for (int j = 0; j < TJ; ++j) {
                     // small cross terms first, the hi*hi product last
                     if constexpr (NSPLIT == 3) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][i], bfr[0][j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfr[1][j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[2][j], acc[i][j], 0, 0, 0);
+                        if constexpr (SA > 2) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[2][i], bfr[0][j], acc[i][j], 0, 0, 0);
+                        if constexpr (SA > 1 && SB > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfr[1][j], acc[i][j], 0, 0, 0);
+                        if constexpr (SB > 2) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[2][j], acc[i][j], 0, 0, 0);
                     }
                     if constexpr (NSPLIT >= 2) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfr[0][j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[1][j], acc[i][j], 0, 0, 0);
+                        if constexpr (SA > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1][i], bfr[0][j], acc[i][j], 0, 0, 0);
+                        if constexpr (SB > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[1][j], acc[i][j], 0, 0, 0);
                     }
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0][i], bfr[0][j], acc[i][j], 0, 0, 0);
                 }
@@ -426,15 +430,27 @@ static int launch_gemm(GemmArgs &a, int precision, int tile, hipStream_t s) {
             break;
         case GTN_PREC_BF16X3:
             if (big)
-                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 2, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
             else
-                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 2, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
             break;
         case GTN_PREC_BF16X6:
             if (big)
-                hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 3, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
             else
-                hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 3, 3, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case GTN_PREC_BF16:
+            if (big)
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 1, 1, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 1, 1, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case GTN_PREC_BF16_SPLIT_A:
+            if (big)
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 1, 128, 128, 32>), dim3(grid), dim3(256), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_gemm_bf16<TIN, 2, 1, 64, 64, 32>), dim3(grid), dim3(256), 0, s, a);
             break;
         default: set_error("gtn_gemm_grouped: unknown precision %d", precision); return GTN_ENOTSUP;
     }

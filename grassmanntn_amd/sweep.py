@@ -59,8 +59,9 @@ class CouplingSweep:
 
     D: leg dim of the generators (seeds `gen_seeds`); steps: TRG steps per point; schedule: the
     per-step arithmetic (see core.mixed_schedule); max_batch(chi) -> largest batch of points of bond
-    chi run together (default: about 24 GiB of step buffers per batch).  prepare(chis) records the
-    graphs of every chi class once; run(beta, m, chi) -> ln Z per site of every point."""
+    chi run together (default: about 24 GiB of step buffers per batch).  prepare(beta, m, chi)
+    records the graphs every batch of those points needs (run() records missing ones itself);
+    run(beta, m, chi) -> ln Z per site of every point."""
 
     def __init__(self, D=32, steps=16, gen_seeds=(10, 11, 12), schedule=None, max_batch=None, graphs=True):
         self.D, self.steps, self.schedule, self.graphs = D, steps, schedule, graphs
@@ -83,30 +84,34 @@ class CouplingSweep:
         _gemm([d], L.GTN_F64, "f64")
         return out
 
-    def _run_for(self, chi, P):
-        key = int(chi)
-        if key not in self.runs:
-            self.runs[key] = BatchedTRG(P, self.D, int(chi), self.steps, schedule=self.schedule, graphs=self.graphs)
-        return self.runs[key]
-
     def batches(self, beta, m, chi):
-        """[(chi, indices of the points of one batch)], points grouped by chi in order of appearance."""
+        """[(chi, batch size, indices of the points of one batch)]: points grouped by chi, at most
+        max_batch(chi) per batch; a batch's size is the smallest power of two (or the maximum) that
+        holds its points, so that few sizes -- and few recorded graphs -- serve any number of points."""
         out = []
         for c in sorted(set(int(x) for x in chi)):
-            idx = np.nonzero(chi == c)[0]
+            idx = np.nonzero(np.asarray(chi) == c)[0]
             B = self.max_batch(c)
             for s in range(0, len(idx), B):
-                out.append((c, idx[s:s + B]))
+                part = idx[s:s + B]
+                P = min(B, 1 << int(np.ceil(np.log2(len(part)))))
+                out.append((c, P, part))
         return out
 
+    def _run_for(self, chi, P, beta=None, m=None, idx=None):
+        key = (int(chi), int(P))
+        if key not in self.runs:
+            run = BatchedTRG(P, self.D, int(chi), self.steps, schedule=self.schedule, graphs=self.graphs)
+            if self.graphs:
+                run.prepare(self._sites(self._coef(beta, m, self._pad(idx, P))))
+            self.runs[key] = run
+        return self.runs[key]
+
     def prepare(self, beta, m, chi):
-        """Warm-up and graph capture of every chi class (with a first batch of its points)."""
-        for c, idx in self.batches(beta, m, chi):
-            if c in self.runs:
-                continue
-            P = min(self.max_batch(c), int(np.sum(chi == c)))
-            run = self._run_for(c, P)
-            run.prepare(self._sites(self._coef(beta, m, self._pad(idx, P))))
+        """Warm-up and graph capture of every (chi, batch size) the points need."""
+        beta, m = np.asarray(beta), np.asarray(m)
+        for c, P, idx in self.batches(beta, m, chi):
+            self._run_for(c, P, beta, m, idx)
 
     @staticmethod
     def _pad(idx, P):
@@ -117,16 +122,23 @@ class CouplingSweep:
     def _coef(beta, m, idx):
         return np.stack([np.ones(len(idx)), beta[idx], m[idx]], axis=1)
 
-    def run(self, beta, m, chi):
-        """ln Z per site (complex where a final supertrace is negative) of every point."""
+    def run(self, beta, m, chi, n_sv=0):
+        """ln Z per site (complex where a final supertrace is negative) of every point; with n_sv > 0
+        also the n_sv leading singular values of every point's final step (T1's blocks, decreasing,
+        divided by the largest; zero-padded), as (lnz, sv)."""
+        beta, m, chi = np.asarray(beta), np.asarray(m), np.asarray(chi)
         out = np.empty(len(beta), dtype=np.complex128)
-        for c, idx in self.batches(beta, m, chi):
-            run = self.runs.get(c) or self._run_for(c, min(self.max_batch(c), int(np.sum(chi == c))))
+        sv = np.zeros((len(beta), n_sv))
+        for c, P, idx in self.batches(beta, m, chi):
+            run = self._run_for(c, P, beta, m, idx)
             full = self._pad(idx, run.P)
             run.start(self._sites(self._coef(beta, m, full)))
             run.advance()
             out[idx] = run.result()[:len(idx)]
-        return out
+            if n_sv:
+                k = min(n_sv, run.final_sv.shape[1])
+                sv[idx, :k] = run.final_sv[:len(idx), :k]
+        return (out, sv) if n_sv else out
 
     def close(self):
         for r in self.runs.values():

@@ -41,6 +41,9 @@ extern "C" {
 #define GTN_PREC_F32 1    /* operands rounded to FP32, FP32 MFMA, FP32 accumulation */
 #define GTN_PREC_BF16X3 2 /* operands split hi+lo in BF16, 3 BF16 MFMA products, FP32 accumulation */
 #define GTN_PREC_BF16X6 3 /* operands split in 3 BF16 parts, 6 products, FP32 accumulation */
+#define GTN_PREC_BF16 4   /* operands rounded to BF16, one product, FP32 accumulation */
+#define GTN_PREC_BF16_SPLIT_A 5 /* A split hi+lo in BF16, B rounded to BF16, 2 products, FP32 accumulation
+                                 * (BF16 weights times activations kept to ~16 bits) */
 
 int gtn_version(void);
 
@@ -283,6 +286,37 @@ typedef struct gtn_svd_opts {
 int64_t gtn_svd_workspace_bytes(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts);
 int gtn_svd_blocks(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts, void *workspace_dev,
                    int64_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Learned surrogate of the coupling sweep (no counterpart in the reference): an ensemble of E MLPs
+ * whose matrix products run on gtn_gemm_grouped (GTN_PREC_BF16_SPLIT_A: BF16 weights, FP32
+ * accumulation) and whose other stages are these kernels.  Activations of a layer are E row-major
+ * (n x c) FP32 matrices one after the other; bias is (E x c).
+ * gtn_mlp_bias_act: y += bias (per column); pre (optional) = y; y = act(y) (GTN_ACT_GELU:
+ *   x Phi(x) = 0.5 x (1 + erf(x / sqrt 2)); GTN_ACT_NONE: identity).
+ * gtn_mlp_act_backward: dy *= GELU'(pre) over count values.
+ * gtn_mlp_colsum: out[e][j] = sum_r x[e][r][j] (bias gradients; FP64 accumulation).
+ * gtn_mlp_mse: with target (n x c) shared by the members and weight (c): dout = 2 w_j (out - t) / n,
+ *   loss[e] = sum w_j (out - t)^2 / n (loss may be NULL).
+ * gtn_adamw: AdamW (decoupled weight decay) on count parameters at step t = step[0] + 1, learning
+ *   rate lr_min + (lr_max - lr_min) (1 + cos(pi (t - 1) / (total_steps - 1))) / 2; then step[0] += 1.
+ * gtn_ensemble_stats: mean and population standard deviation over the E members (sd may be NULL).
+ * ---------------------------------------------------------------------------------------------- */
+#define GTN_ACT_NONE 0
+#define GTN_ACT_GELU 1
+typedef struct gtn_adamw_opts {
+    double lr_max, lr_min, beta1, beta2, eps, weight_decay;
+    int64_t total_steps;
+} gtn_adamw_opts;
+int gtn_mlp_bias_act(float *y_dev, int64_t E, int64_t n, int64_t c, const float *bias_dev, int act, float *pre_dev,
+                     void *stream);
+int gtn_mlp_act_backward(float *dy_dev, const float *pre_dev, int64_t count, int act, void *stream);
+int gtn_mlp_colsum(const float *x_dev, int64_t E, int64_t n, int64_t c, float *out_dev, void *stream);
+int gtn_mlp_mse(const float *out_dev, const float *target_dev, const float *weight_dev, int64_t E, int64_t n, int64_t c,
+                float *dout_dev, double *loss_dev, void *stream);
+int gtn_adamw(float *w_dev, const float *g_dev, float *m_dev, float *v_dev, int64_t count, const gtn_adamw_opts *opts,
+              double *step_dev, void *stream);
+int gtn_ensemble_stats(const float *out_dev, int64_t E, int64_t count, float *mean_dev, float *sd_dev, void *stream);
 
 #ifdef __cplusplus
 }

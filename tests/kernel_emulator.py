@@ -288,7 +288,15 @@ class FakeLib:
                     Bv = _view(b0, kb + 1, dt)
                     Am = np.lib.stride_tricks.as_strided(Av, (d.M, d.K), (d.sam * es, d.sak * es))
                     Bm = np.lib.stride_tricks.as_strided(Bv, (d.K, d.N), (d.sbk * es, d.sbn * es))
-                    prod = Am @ Bm
+                    if prec in (4, 5):
+                        # BF16 operands (mode 5: the first split hi + lo), products accumulated wider
+                        from oracle.mlp_oracle import bf16
+                        A64 = Am.astype(np.float64)
+                        hi = bf16(A64)
+                        A2 = hi + bf16(A64 - hi) if prec == 5 else hi
+                        prod = (A2 @ bf16(Bm.astype(np.float64))).astype(dt)
+                    else:
+                        prod = Am @ Bm
                 else:
                     prod = np.zeros((d.M, d.N), dtype=dt)
                 Cm = np.lib.stride_tricks.as_strided(Cv, (d.M, d.N), (d.scm * es, d.scn * es))
@@ -389,6 +397,61 @@ class FakeLib:
         out[...] = 0
         r = np.arange(2 * d)
         out[:, r, r] = sv[:, r % 2, r // 2]
+        return 0
+
+    # ---- MLP stages (the formulas of include/gtn_hip.h; FP32 storage) ---------------------------------
+    def gtn_mlp_bias_act(self, y, E, n, c, bias, act, pre, stream):
+        from oracle.mlp_oracle import gelu
+        Y = self._arr(y, E * n * c, np.float32, "bias_act y").reshape(E, n, c)
+        v = Y.astype(np.float64) + self._arr(bias, E * c, np.float32, "bias").reshape(E, 1, c)
+        if pre:
+            self._arr(pre, E * n * c, np.float32, "bias_act pre")[:] = v.reshape(-1)
+        Y[...] = gelu(v) if act == 1 else v
+        return 0
+
+    def gtn_mlp_act_backward(self, dy, pre, count, act, stream):
+        from oracle.mlp_oracle import gelu_grad
+        d = self._arr(dy, count, np.float32, "act_backward dy")
+        d[:] = d * gelu_grad(self._arr(pre, count, np.float32, "act_backward pre").astype(np.float64))
+        return 0
+
+    def gtn_mlp_colsum(self, x, E, n, c, out, stream):
+        X = self._arr(x, E * n * c, np.float32, "colsum x").reshape(E, n, c).astype(np.float64)
+        self._arr(out, E * c, np.float32, "colsum out")[:] = X.sum(axis=1).reshape(-1)
+        return 0
+
+    def gtn_mlp_mse(self, out, target, weight, E, n, c, dout, loss, stream):
+        O_ = self._arr(out, E * n * c, np.float32, "mse out").reshape(E, n, c).astype(np.float64)
+        T = self._arr(target, n * c, np.float32, "mse target").reshape(n, c).astype(np.float64)
+        w = self._arr(weight, c, np.float32, "mse weight").astype(np.float64)
+        d = O_ - T[None]
+        self._arr(dout, E * n * c, np.float32, "mse dout")[:] = (2.0 * w * d / n).reshape(-1)
+        if loss:
+            self._arr(loss, E, np.float64, "mse loss")[:] = (w * d * d).sum(axis=(1, 2)) / n
+        return 0
+
+    def gtn_adamw(self, w, g, m, v, count, opts, step, stream):
+        o = opts._obj
+        st = self._arr(step, 1, np.float64, "adamw step")
+        t = st[0] + 1.0
+        frac = min((t - 1.0) / (o.total_steps - 1), 1.0) if o.total_steps > 1 else 0.0
+        lr = o.lr_min + 0.5 * (o.lr_max - o.lr_min) * (1.0 + np.cos(np.pi * frac))
+        W = self._arr(w, count, np.float32, "adamw w")
+        G = self._arr(g, count, np.float32, "adamw g").astype(np.float64)
+        Mv = self._arr(m, count, np.float32, "adamw m")
+        Vv = self._arr(v, count, np.float32, "adamw v")
+        mi = o.beta1 * Mv + (1 - o.beta1) * G
+        vi = o.beta2 * Vv + (1 - o.beta2) * G * G
+        Mv[:], Vv[:] = mi, vi
+        W[:] = W * (1 - lr * o.weight_decay) - lr * (mi / (1 - o.beta1 ** t)) / (np.sqrt(vi / (1 - o.beta2 ** t)) + o.eps)
+        st[0] += 1.0
+        return 0
+
+    def gtn_ensemble_stats(self, out, E, count, mean, sd, stream):
+        X = self._arr(out, E * count, np.float32, "ens out").reshape(E, count).astype(np.float64)
+        self._arr(mean, count, np.float32, "ens mean")[:] = X.mean(axis=0)
+        if sd:
+            self._arr(sd, count, np.float32, "ens sd")[:] = X.std(axis=0)
         return 0
 
     def gtn_gemm_grouped_noalloc(self, groups, n, dtype, prec, stream):
