@@ -1,20 +1,25 @@
 #!/usr/bin/env python
-"""TRG coarse-graining throughput on one MI355X: mixed precision vs FP64.
+"""TRG coarse-graining throughput on one MI355X: mixed precision vs FP64, batched coupling sweeps
+and the learned surrogate.
 
-Workload (leg dim 64, chi 64, 20 steps): a seeded Grassmann-even 4-leg site tensor (entries N(0,1)
-times exp(-0.5 * sum of leg popcounts), odd total parity zeroed, max|T| = 1, parity-preserving legs,
-statistics (1,1,-1,-1)) coarse-grained by Levin-Nave TRG for 20 steps.  A step is one gtn.trg call
-(two signed permutations, two parity-block SVDs, six contractions, the norm).  `--warmup W` runs W
-untimed steps first (from the same tensor, results discarded); the timed region is then `--steps K`
-steps from the initial tensor, so that ln Z per site after K <= 20 steps is always checked against the
-FP64 CPU oracle's golden values.
+Headline (value): TRG steps/s at leg dim 64, chi 64 of the mixed-precision batched path -- P seeded
+Grassmann-even site tensors (entries N(0,1) times exp(-0.5 * sum of leg popcounts), odd total parity
+zeroed, max|T| = 1, parity-preserving legs, statistics (1,1,-1,-1); seeds 1..P) coarse-grained
+together for --steps (20) steps, every step one replay of a recorded HIP graph; arithmetic: the first
+five steps FP64, then contractions in FP32 MFMA with FP32 accumulation, while the parity-block SVD
+(range finder included), the normalisation and the log-norm sum stay FP64 at every step.  Steps/s
+counts (point, step) pairs.  Beside it: the plain FP64 GPU path (one tensor, the eager gtn.trg loop)
+and the FP64 batched path (same batch, all FP64), ln Z per site of every point with an oracle golden
+against that golden, the roofline of the step's dominant kernel (the range finder's FP64 GEMM, timed
+with HIP events on the same shape), the coupling sweep (512 points, leg-dim-32 generators, chi from
+{16, 24, 32, 48, 64}, 16 steps: ln Z points/s), the learned surrogate (a 5-member MLP ensemble
+trained on 8192 TRG-labelled points at chi 32, queried on 4096 points with TRG fallback where the
+ensemble spread exceeds 1e-4: ln Z points/s and its error against TRG), the standalone
+'ijkl,klmn->ijmn' contraction of two 64^4 tensors and the CPU baseline (the numpy oracle).
 
-Prints one JSON line: value = steps/s of the mixed-precision path (the first five steps in FP64, then
-contractions in FP32 MFMA with FP32 accumulation; the parity-block SVD -- range finder included -- the
-normalisation and the log-norm sum in FP64 at every step), with the FP64 path's steps/s, ln Z per site of both against the oracle, the number of
-steps recomputed after an assumed rank count failed, the roofline of the T' contraction GEMM (time from
-HIP events around each launch; HBM bytes from the committed rocprofv3 counter summary) and the CPU
-baseline (the numpy oracle timed on the host).
+`--warmup W` replays W untimed steps (and the plain path runs W untimed eager steps) before timing;
+every timed run starts from the initial tensors, so that ln Z after the 20 steps of the workload is
+always checked against the oracle.
 """
 import argparse
 import json
@@ -31,23 +36,33 @@ sys.path.insert(0, ROOT)
 D, CHI, SEED = 64, 64, 1
 MFMA_PEAK_TF = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
 SPLIT_PRODUCTS = {"bf16x3": 3, "bf16x6": 6}
+GOLDEN_SEEDS = {1: "d64_chi64", 2: "d64_chi64_s2", 3: "d64_chi64_s3", 4: "d64_chi64_s4"}
 
 
 def site_tensor(d, seed):
     """The synthetic workload's site tensor on the host (numpy)."""
-    rng = np.random.default_rng(seed)
-    X = rng.standard_normal((d, d, d, d))
-    i = np.arange(d)
-    pc = lambda x: np.array([bin(int(v)).count("1") for v in x])
-    enc = i ^ (pc(i >> 1) & 1)
-    w = np.exp(-0.5 * pc(enc))
-    par = i & 1
-    mask = (par[:, None, None, None] ^ par[None, :, None, None] ^ par[None, None, :, None] ^ par[None, None, None, :]) == 0
-    T = X * w[:, None, None, None] * w[None, :, None, None] * w[None, None, :, None] * w[None, None, None, :] * mask
-    return T / np.abs(T).max()
+    from grassmanntn_amd.sweep import site_array
+    return site_array(d, seed)
+
+
+def golden_lnz(n_steps):
+    """{seed: ln Z per site after n_steps} from the committed oracle goldens (leg dim 64, chi 64)."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "trg_golden.json")))
+    out = {}
+    for seed, name in GOLDEN_SEEDS.items():
+        e = g.get(name)
+        if e and n_steps <= len(e["lnz_per_site_real"]):
+            out[seed] = complex(e["lnz_per_site_real"][n_steps - 1], e["lnz_per_site_imag"][n_steps - 1])
+    return out
 
 
 VERBOSE = False
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """A progress line on stderr (the run's stages take seconds to a minute each)."""
+    print("[bench %6.1fs] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
 
 
 def einsum_microbench(gtn, prec, reps=10):
@@ -180,6 +195,151 @@ def copy_calibration(mib, reps=5):
                                            "bytes_written_per_launch": rows * cols * 8, "launches": reps}}))
 
 
+def batched_path(gtn, P, steps, warmup, schedule, T0s):
+    """P points of leg dim 64 at chi 64 through the graph-replayed batched TRG: graphs recorded
+    (untimed), `warmup` untimed replays, then the timed run of `steps` steps from T0s (device-resident
+    before timing).  Returns timings and ln Z per site of every point."""
+    from grassmanntn_amd._device import DeviceArray, Event, synchronize
+    from grassmanntn_amd.batch import BatchedTRG
+    t0 = time.perf_counter()
+    run = BatchedTRG(P, D, CHI, steps, schedule=schedule)
+    X0 = DeviceArray.from_numpy(T0s.reshape(P, -1))
+    run.prepare(X0)
+    run.start(X0)
+    for i in range(warmup):
+        run.graphs[run._key(min(i, steps - 1))].launch()
+    synchronize()
+    prep = time.perf_counter() - t0
+    e0, e1 = Event(), Event()
+    w0 = time.perf_counter()
+    e0.record()
+    run.start(X0)
+    run.advance()
+    e1.record()
+    synchronize()
+    wall = time.perf_counter() - w0
+    ms = e0.elapsed_ms(e1)
+    lnz = run.result()
+    run.close()
+    return {"gpu_ms": ms, "wall_s": wall, "prepare_s": prep, "lnz": lnz}
+
+
+def rf_gemm_roofline(P, reps=20):
+    """The batched step's dominant kernel: the range finder's FP64 GEMM of its parity blocks (4P blocks
+    of 2048 x 2048 times 2048 x 128, one launch), run `reps` times on the same shape and timed with
+    HIP events on the library's stream.  Algorithmic work 2 M N K per block."""
+    import ctypes as C
+    from grassmanntn_amd import _lib as L
+    from grassmanntn_amd._device import DeviceArray, Event, synchronize
+    from grassmanntn_amd.core import _gemm_desc
+    nb, m, k, p = 4 * P, D * D // 2, D * D // 2, 128
+    A = DeviceArray.zeros((nb, m, k))
+    B = DeviceArray.zeros((nb, k, p))
+    Cm = DeviceArray.empty((nb, m, p))
+    d = _gemm_desc(A.data_ptr(), k, 1, B.data_ptr(), p, 1, Cm.data_ptr(), p, 1, m, p, k, batch=nb,
+                   sab=m * k, sbb=k * p, scb=m * p)
+    arr = (L.GemmDesc * 1)(d)
+    lib = L.lib()
+    for _ in range(2):
+        L.check(lib.gtn_gemm_grouped(arr, 1, L.GTN_F64, L.PREC["f64"], L.stream_ptr()), "gemm")
+    synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(reps):
+        L.check(lib.gtn_gemm_grouped(arr, 1, L.GTN_F64, L.PREC["f64"], L.stream_ptr()), "gemm")
+    e1.record()
+    synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    flop = 2.0 * m * p * k * nb
+    ach = flop / (ms / 1000.0) / 1e12
+    return {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TF["f64"], "unit": "TFLOP/s",
+            "frac": ach / MFMA_PEAK_TF["f64"], "traffic": measured_traffic("rf_gemm_f64"),
+            "kernel": "range-finder GEMM k_gemm_fp<double,double,128,128,16>: %d parity blocks of 2048 x 2048 "
+                      "times 2048 x 128 per launch, FP64" % nb,
+            "flop_per_launch": flop, "avg_launch_ms": ms, "launches": reps}
+
+
+def coupling_sweep(gtn):
+    """The 512-point coupling sweep (leg-dim-32 generators, seeds 10/11/12; beta, m, chi from seed 100;
+    16 steps, mixed schedule): ln Z points/s, and the points of the committed oracle subset."""
+    from grassmanntn_amd._device import synchronize
+    from grassmanntn_amd.sweep import CouplingSweep, coupling_points
+    beta, m, chi = coupling_points(512, 100)
+    sw = CouplingSweep(D=32, steps=16, gen_seeds=(10, 11, 12), schedule=lambda i: gtn.mixed_schedule(i, "f32"))
+    t0 = time.perf_counter()
+    sw.prepare(beta, m, chi)
+    synchronize()
+    prep = time.perf_counter() - t0
+    progress("coupling sweep: graphs recorded (%.1f s)" % prep)
+    t0 = time.perf_counter()
+    lnz = sw.run(beta, m, chi, progress=progress)
+    synchronize()
+    wall = time.perf_counter() - t0
+    sw.close()
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "sweep_golden.json")))
+    errs = {str(p["chi"]): abs(lnz[p["index"]] - complex(*p["lnz_per_site"])) for p in g["points"]}
+    counts = {str(c): int(np.sum(chi == c)) for c in sorted(set(int(x) for x in chi))}
+    return {"points": 512, "steps": 16, "wall_s": wall, "prepare_s": prep, "lnz_points_per_s": 512 / wall,
+            "point_steps_per_s": 512 * 16 / wall, "points_per_chi": counts,
+            "dlnz_vs_oracle_by_chi": errs, "dlnz_vs_oracle_max": max(errs.values())}
+
+
+def surrogate_workload(gtn, n_train=8192, n_query=4096, tol=1e-4, train_steps=12000):
+    """The learned surrogate: TRG labels at chi 32 for n_train points (seed 101), a 5-member ensemble
+    trained on them, then n_query points (seed 102) answered by the surrogate with TRG fallback where
+    the ensemble spread exceeds tol, against TRG on the same points."""
+    from grassmanntn_amd._device import synchronize
+    from grassmanntn_amd.sweep import CouplingSweep
+    from grassmanntn_amd.surrogate import Surrogate
+    rt, rq = np.random.default_rng(101), np.random.default_rng(102)
+    bt, mt = rt.uniform(0.1, 2.0, n_train), rt.uniform(0.0, 1.0, n_train)
+    bq, mq = rq.uniform(0.1, 2.0, n_query), rq.uniform(0.0, 1.0, n_query)
+    sw = CouplingSweep(D=32, steps=16, gen_seeds=(10, 11, 12), schedule=lambda i: gtn.mixed_schedule(i, "f32"))
+    sur = Surrogate(sw, chi=32, members=5, seed=0, tol=tol)
+    sw.prepare(bt[:1024], mt[:1024], np.full(1024, 32))
+    progress("surrogate: TRG labels of %d points" % n_train)
+    t0 = time.perf_counter()
+    Y = sur.labels(bt, mt)
+    synchronize()
+    t_label = time.perf_counter() - t0
+    progress("surrogate: training (%d steps)" % train_steps)
+    t0 = time.perf_counter()
+    losses = sur.train(bt, mt, Y, steps=train_steps, lr=3e-3, lr_min=3e-6, lnz_weight=256.0)
+    synchronize()
+    t_train = time.perf_counter() - t0
+    progress("surrogate: TRG on %d queries" % n_query)
+    t0 = time.perf_counter()
+    truth = sw.run(bq, mq, np.full(n_query, 32)).real
+    synchronize()
+    t_trg = time.perf_counter() - t0
+    progress("surrogate: queries with fallback")
+    t0 = time.perf_counter()
+    lnz, fb, t_nn, t_fb = sur.query(bq, mq)
+    t_q = time.perf_counter() - t0
+    mean, sd = sur.predict(bq, mq)
+    sw.close()
+    err = np.abs(lnz - truth)
+    err_nn = np.abs(mean[:, 0] - truth)
+    acc = ~fb
+    table = {}
+    for t in (1e-4, 3e-4, 1e-3, 3e-3):
+        ok = sd[:, 0] <= t
+        table[str(t)] = {"accepted_frac": float(ok.mean()),
+                         "max_err_accepted": float(err_nn[ok].max()) if ok.any() else None,
+                         "frac_accepted_err_over_tol": float((err_nn[ok] > t).mean()) if ok.any() else None}
+    return {"train_points": n_train, "queries": n_query, "chi": 32, "tol": tol,
+            "label_s": t_label, "label_points_per_s": n_train / t_label, "train_s": t_train,
+            "train_steps": train_steps, "final_member_losses": [float(x) for x in losses],
+            "trg_query_s": t_trg, "trg_lnz_points_per_s": n_query / t_trg,
+            "surrogate_query_s": t_q, "surrogate_lnz_points_per_s": n_query / t_q,
+            "surrogate_gain_vs_trg": (n_query / t_q) / (n_query / t_trg),
+            "inference_only_points_per_s": n_query / t_nn if t_nn > 0 else None,
+            "fallback_frac": float(fb.mean()), "max_err_with_fallback": float(err.max()),
+            "accepted_max_err": float(err[acc].max()) if acc.any() else None,
+            "surrogate_err_median": float(np.median(err_nn)), "surrogate_err_p90": float(np.percentile(err_nn, 90)),
+            "tolerance_table": table}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,15 +352,16 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5,
                     help="leading TRG steps of the mixed path run in FP64 (their normalisation errors "
                          "weigh 2^-n in ln Z per site)")
+    ap.add_argument("--batch", type=int, default=16, help="points coarse-grained together in the headline path")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 512-point coupling sweep")
+    ap.add_argument("--no-c4", action="store_true", help="skip the learned surrogate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-einsum", action="store_true", help="skip the standalone 'ijkl,klmn->ijmn' "
                     "contraction of two 64^4 tensors")
     ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
     ap.add_argument("--svd-iters", type=int, default=0, help="subspace iterations of the block SVD "
                     "(0: the library default)")
-    ap.add_argument("--mixed-svd-iters", type=int, default=0, help="subspace iterations of the block "
-                    "SVD in the mixed path (0: as the FP64 path)")
     ap.add_argument("--svd-orth-every", type=int, default=0, help="re-orthonormalise the range-finder "
                     "basis every this many iterations (0: the library default)")
     ap.add_argument("--svd-oversample", type=int, default=-1, help="range-finder columns beyond the kept "
@@ -222,85 +383,88 @@ def main():
         gtn.settings.svd_orth_every = args.svd_orth_every
     if args.svd_oversample >= 0:
         gtn.settings.svd_oversample = args.svd_oversample
-    T0 = site_tensor(D, SEED)
-    fp64_iters = gtn.settings.svd_iters
-    mixed_iters = args.mixed_svd_iters or fp64_iters
-    gtn.settings.svd_iters = mixed_iters
-    mixed = run_path(gtn, args.precision, args.svd_precision, args.steps, args.warmup, T0, args.fp64_steps)
-    gtn.settings.svd_iters = fp64_iters
-    fp64 = None if args.no_fp64 else run_path(gtn, "f64", "f64", args.steps, args.warmup, T0)
-    golden = None
-    gpath = os.path.join(ROOT, "tests", "golden", "trg_golden.json")
-    n_total = args.steps
-    if os.path.exists(gpath):
-        g = json.load(open(gpath)).get("d64_chi64")
-        if g and n_total <= len(g["lnz_per_site_real"]):
-            golden = complex(g["lnz_per_site_real"][n_total - 1], g["lnz_per_site_imag"][n_total - 1])
-    sps = args.steps / (mixed["gpu_ms"] / 1000.0)
-    tp = mixed["tprime_ms"]
-    flop = 2 * 2.0 * (CHI * CHI // 2) * (D * D // 2) * (CHI * CHI // 2)
-    avg_ms = sum(tp) / len(tp) if tp else float("nan")
-    achieved = flop / (avg_ms / 1000.0) / 1e12 if tp else None
-    peak = MFMA_PEAK_TF[args.precision]
+    P, K = args.batch, args.steps
+    prec = args.precision
+    mixed_sched = lambda i: ("f64", "f64") if i < args.fp64_steps else (prec, args.svd_precision)
+    T0s = np.stack([site_tensor(D, SEED + s) for s in range(P)])
+    golden = golden_lnz(K)
+    progress("mixed batched path: %d points, %d steps" % (P, K))
+    mixed = batched_path(gtn, P, K, args.warmup, mixed_sched, T0s)
+    progress("mixed batched path: %.1f ms" % mixed["gpu_ms"])
+    value = P * K / (mixed["gpu_ms"] / 1000.0)
     out = {
-        "metric": "TRG steps/s at chi=64 (mixed precision), leg dim 64",
-        "value": sps,
+        "metric": "TRG steps/s at chi=64 and ln Z points/s, mixed vs FP64 GPU at |dlnZ|<=1e-8",
+        "value": value,
         "unit": "TRG steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": mixed["gpu_ms"] / args.steps,
+        "ms_per_step": mixed["gpu_ms"] / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "%s contractions, %s range finder, f64 SVD/normalisation (first %d steps all f64)" % (
-            args.precision, args.svd_precision, args.fp64_steps),
+        "dtype": "f64 for steps 1-%d, then %s contractions; f64 SVD, normalisation and log-norm sum" % (
+            args.fp64_steps, prec),
         "data": "synthetic",
-        "config": {"workload": "TRG, leg dim 64, chi 64, %d steps from the seed-%d site tensor" % (n_total, SEED),
-                   "leg_dim": D, "chi": CHI, "steps_total": n_total, "svd_iters": mixed_iters,
-                   "svd_iters_fp64_path": fp64_iters, "svd_oversample": gtn.settings.svd_oversample,
-                   "svd_orth_every": gtn.settings.svd_orth_every},
-        "lnz_per_site_mixed": [mixed["lnz"].real, mixed["lnz"].imag],
-        "recomputed_steps_mixed": mixed["recomputed_steps"],
-        "recompute_misses_mixed": mixed["recompute_misses"],
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": (achieved / peak) if achieved else None,
-                     "traffic": measured_traffic("tprime_gemm_" + args.precision),
-                     "kernel": "T' contraction, 2 parity-block GEMMs of 2048^3, %s" % args.precision,
-                     "flop_per_launch": flop, "avg_launch_ms": avg_ms, "launches": len(tp)},
+        "config": {"workload": "TRG at leg dim 64, chi 64: %d seeded site tensors (seeds %d..%d) coarse-grained "
+                               "together for %d steps, one HIP-graph replay per step" % (P, SEED, SEED + P - 1, K),
+                   "leg_dim": D, "chi": CHI, "points": P, "steps": K, "svd_iters": gtn.settings.svd_iters,
+                   "svd_oversample": gtn.settings.svd_oversample, "svd_orth_every": gtn.settings.svd_orth_every},
+        "ms_per_point_step_mixed": mixed["gpu_ms"] / (P * K),
+        "graph_prepare_s": mixed["prepare_s"],
     }
-    terms = SPLIT_PRODUCTS.get(args.precision)
-    if terms and achieved:
-        # split-BF16 forms issue `terms` BF16 matrix-core products per FP64-equivalent product
-        out["roofline"]["matrix_core_products_per_flop"] = terms
-        out["roofline"]["matrix_core_tflops"] = terms * achieved
-        out["roofline"]["matrix_core_frac"] = terms * achieved / peak
-    if fp64 is not None:
-        out["fp64_steps_per_s"] = args.steps / (fp64["gpu_ms"] / 1000.0)
-        out["speedup_mixed_vs_fp64"] = out["value"] / out["fp64_steps_per_s"]
-        out["lnz_per_site_fp64"] = [fp64["lnz"].real, fp64["lnz"].imag]
-        out["recomputed_steps_fp64"] = fp64["recomputed_steps"]
-        out["dlnz_mixed_vs_fp64_gpu"] = abs(mixed["lnz"] - fp64["lnz"])
-        if fp64["tprime_ms"]:
-            ms64 = sum(fp64["tprime_ms"]) / len(fp64["tprime_ms"])
-            a64 = flop / (ms64 / 1000.0) / 1e12
-            out["fp64_tprime_tflops"] = a64
-            out["roofline_fp64_path"] = {"bound": "mfma", "achieved": a64, "peak": MFMA_PEAK_TF["f64"],
-                                         "unit": "TFLOP/s", "frac": a64 / MFMA_PEAK_TF["f64"],
-                                         "traffic": measured_traffic("tprime_gemm_f64"),
-                                         "kernel": "T' contraction, 2 parity-block GEMMs of 2048^3, f64",
-                                         "avg_launch_ms": ms64, "launches": len(fp64["tprime_ms"])}
-    if golden is not None:
-        out["oracle_check"] = "ln Z per site after %d steps vs tests/golden/trg_golden.json d64_chi64" % n_total
-        out["dlnz_mixed_vs_oracle"] = abs(mixed["lnz"] - golden)
-        if fp64 is not None:
-            out["dlnz_fp64_vs_oracle"] = abs(fp64["lnz"] - golden)
+    if golden:
+        errs = {str(s): abs(mixed["lnz"][s - SEED] - g) for s, g in golden.items() if s - SEED < P}
+        out["oracle_check"] = "ln Z per site after %d steps of the points with seeds %s vs tests/golden/trg_golden.json" % (
+            K, sorted(int(x) for x in errs))
+        out["dlnz_mixed_vs_oracle_by_seed"] = errs
+        out["dlnz_mixed_vs_oracle"] = max(errs.values())
     else:
-        out["oracle_check"] = "skipped: the golden covers 20 steps, %d were timed" % n_total
+        out["oracle_check"] = "skipped: the goldens cover 20 steps, %d were timed" % K
+    if not args.no_fp64:
+        progress("FP64 batched path")
+        f64b = batched_path(gtn, P, K, args.warmup, lambda i: ("f64", "f64"), T0s)
+        out["fp64_batched_steps_per_s"] = P * K / (f64b["gpu_ms"] / 1000.0)
+        out["speedup_mixed_vs_fp64_batched"] = value / out["fp64_batched_steps_per_s"]
+        if golden:
+            out["dlnz_fp64_batched_vs_oracle"] = max(abs(f64b["lnz"][s - SEED] - g) for s, g in golden.items()
+                                                     if s - SEED < P)
+        progress("plain FP64 path")
+        plain = run_path(gtn, "f64", "f64", K, args.warmup, T0s[0])
+        out["fp64_steps_per_s"] = K / (plain["gpu_ms"] / 1000.0)
+        out["speedup_mixed_vs_fp64"] = value / out["fp64_steps_per_s"]
+        out["fp64_path"] = "plain FP64 GPU path: one tensor (seed %d), the eager gtn.trg loop, %d steps" % (SEED, K)
+        out["recomputed_steps_fp64"] = plain["recomputed_steps"]
+        if SEED in golden:
+            out["dlnz_fp64_vs_oracle"] = abs(plain["lnz"] - golden[SEED])
+        if plain["tprime_ms"]:
+            flop = 2 * 2.0 * (CHI * CHI // 2) * (D * D // 2) * (CHI * CHI // 2)
+            ms64 = sum(plain["tprime_ms"]) / len(plain["tprime_ms"])
+            a64 = flop / (ms64 / 1000.0) / 1e12
+            out["roofline_tprime_fp64"] = {"bound": "mfma", "achieved": a64, "peak": MFMA_PEAK_TF["f64"],
+                                           "unit": "TFLOP/s", "frac": a64 / MFMA_PEAK_TF["f64"],
+                                           "traffic": measured_traffic("tprime_gemm_f64"),
+                                           "kernel": "T' contraction, 2 parity-block GEMMs of 2048^3, f64 (plain path)",
+                                           "avg_launch_ms": ms64, "launches": len(plain["tprime_ms"])}
+    progress("range-finder GEMM roofline")
+    out["roofline"] = rf_gemm_roofline(P)
+    if not args.no_c3:
+        progress("coupling sweep: 512 points")
+        out["coupling_sweep"] = coupling_sweep(gtn)
+        out["lnz_points_per_s_trg_sweep"] = out["coupling_sweep"]["lnz_points_per_s"]
+    if not args.no_c4:
+        progress("surrogate: labels, training, queries")
+        sur = surrogate_workload(gtn)
+        out["surrogate"] = sur
+        out["lnz_points_per_s"] = sur["surrogate_lnz_points_per_s"]
+        out["lnz_points_per_s_trg"] = sur["trg_lnz_points_per_s"]
     if not args.no_einsum:
-        out["einsum_ijkl_klmn_d64"] = [einsum_microbench(gtn, "f64"), einsum_microbench(gtn, args.precision)]
+        progress("einsum 'ijkl,klmn->ijmn' at 64^4")
+        out["einsum_ijkl_klmn_d64"] = [einsum_microbench(gtn, "f64"), einsum_microbench(gtn, prec)]
     if not args.no_cpu_baseline:
+        progress("CPU baseline (numpy oracle)")
         out["cpu_baseline"] = cpu_baseline()
+    progress("done")
     print(json.dumps(out))
 
 

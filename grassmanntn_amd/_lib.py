@@ -129,7 +129,8 @@ def _bind(lib):
         "gtn_diag_interleave": (C.c_int, [vp, i64, i64, vp, i32, vp]),
         "gtn_mlp_bias_act": (C.c_int, [vp, i64, i64, i64, vp, i32, vp, vp]),
         "gtn_mlp_act_backward": (C.c_int, [vp, vp, i64, i32, vp]),
-        "gtn_mlp_colsum": (C.c_int, [vp, i64, i64, i64, vp, vp]),
+        "gtn_mlp_colsum": (C.c_int, [vp, i64, i64, i64, vp, vp, vp]),
+        "gtn_mlp_colsum_workspace": (C.c_int64, [i64, i64]),
         "gtn_mlp_mse": (C.c_int, [vp, vp, vp, i64, i64, i64, vp, vp, vp]),
         "gtn_adamw": (C.c_int, [vp, vp, vp, vp, i64, C.POINTER(AdamwOpts), vp, vp]),
         "gtn_ensemble_stats": (C.c_int, [vp, i64, i64, vp, vp, vp]),

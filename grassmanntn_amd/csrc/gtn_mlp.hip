@@ -37,14 +37,32 @@ __global__ void k_act_backward(float *dy, const float *pre, int64_t count) {
         dy[i] *= gelu_grad(pre[i]);
 }
 
-// out[e][j] = sum_r x[e][r][j]: one thread per (e, j), FP64 accumulation, fixed order
-__global__ void k_colsum(const float *x, int64_t E, int64_t n, int64_t c, float *out) {
+// out[e][j] = sum_r x[e][r][j] in two passes with a fixed summation order (FP64 accumulation).
+// Pass 1, grid (ceil(c / 64), E, kColChunks): rows [q n / R, (q + 1) n / R) of 64 columns; lane ->
+// column, the four wavefronts take every fourth row; part[(e R + q) c + j].
+constexpr int kColChunks = 32;
+__global__ void __launch_bounds__(256) k_colsum_partial(const float *x, int64_t n, int64_t c, double *part) {
+    __shared__ double sh[4][64];
+    const int64_t e = blockIdx.y, q = blockIdx.z, j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int w = threadIdx.x >> 6;
+    const int64_t r0 = q * n / kColChunks, r1 = (q + 1) * n / kColChunks;
+    double acc = 0.0;
+    if (j < c) {
+        const float *xe = x + e * n * c + j;
+#pragma unroll 4
+        for (int64_t r = r0 + w; r < r1; r += 4) acc += (double)xe[r * c];
+    }
+    sh[w][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (w == 0 && j < c)
+        part[(e * kColChunks + q) * c + j] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+__global__ void k_colsum_final(const double *part, int64_t E, int64_t c, float *out) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= E * c) return;
     const int64_t e = t / c, j = t - e * c;
-    const float *xe = x + e * n * c + j;
     double acc = 0.0;
-    for (int64_t r = 0; r < n; ++r) acc += (double)xe[r * c];
+    for (int q = 0; q < kColChunks; ++q) acc += part[(e * kColChunks + q) * c + j];
     out[t] = (float)acc;
 }
 
@@ -137,9 +155,15 @@ int gtn_mlp_act_backward(float *dy, const float *pre, int64_t count, int act, vo
     return GTN_OK;
 }
 
-int gtn_mlp_colsum(const float *x, int64_t E, int64_t n, int64_t c, float *out, void *stream) {
-    GTN_REQUIRE(E >= 1 && n >= 0 && c >= 1 && x && out, "gtn_mlp_colsum: bad argument");
-    hipLaunchKernelGGL(k_colsum, dim3(grid_for(E * c, 256)), dim3(256), 0, as_stream(stream), x, E, n, c, out);
+int64_t gtn_mlp_colsum_workspace(int64_t E, int64_t c) { return E * c * kColChunks * (int64_t)sizeof(double); }
+
+int gtn_mlp_colsum(const float *x, int64_t E, int64_t n, int64_t c, float *out, double *part, void *stream) {
+    GTN_REQUIRE(E >= 1 && E <= 65535 && n >= 0 && c >= 1 && x && out && part, "gtn_mlp_colsum: bad argument");
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_colsum_partial, dim3((unsigned)((c + 63) / 64), (unsigned)E, kColChunks), dim3(256), 0, s, x, n,
+                       c, part);
+    GTN_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_colsum_final, dim3(grid_for(E * c, 256)), dim3(256), 0, s, (const double *)part, E, c, out);
     GTN_CHECK_LAUNCH();
     return GTN_OK;
 }

@@ -76,10 +76,13 @@ class MLPEnsemble:
         if self._n == n:
             return
         self._n = n
-        self.Z = [DeviceArray.empty((self.E, n, self.dims[l + 1]), np.float32) for l in range(self.L)]
-        self.H = [DeviceArray.empty((self.E, n, self.dims[l + 1]), np.float32) for l in range(self.L - 1)]
+        # A[l]: layer l's output (GEMM result, then bias and GELU in place); P[l]: its pre-activation
+        self.A = [DeviceArray.empty((self.E, n, self.dims[l + 1]), np.float32) for l in range(self.L)]
+        self.P = [DeviceArray.empty((self.E, n, self.dims[l + 1]), np.float32) for l in range(self.L - 1)]
         self.dZ = [DeviceArray.empty((self.E, n, self.dims[l + 1]), np.float32) for l in range(self.L)]
         self.loss = DeviceArray.zeros(self.E)
+        cmax = max(self.dims[1:])
+        self.colpart = DeviceArray.empty(L.lib().gtn_mlp_colsum_workspace(self.E, cmax) // 8)
 
     def _mm(self, A, sam, sak, sab, B, sbk, sbn, sbb, Cm, scm, M, N, K, scb):
         d = _gemm_desc(A.data_ptr(), sam, sak, B.data_ptr(), sbk, sbn, Cm.data_ptr(), scm, 1, M, N, K,
@@ -95,16 +98,13 @@ class MLPEnsemble:
         H, hshared = X, True
         for l in range(self.L):
             a, b = self.dims[l], self.dims[l + 1]
-            Z = self.Z[l]
-            self._mm(H, a, 1, 0 if hshared else n * a, self.W(l), b, 1, a * b, Z, b, n, b, a, n * b)
+            A = self.A[l]
+            self._mm(H, a, 1, 0 if hshared else n * a, self.W(l), b, 1, a * b, A, b, n, b, a, n * b)
             last = l == self.L - 1
-            out = Z if last else self.H[l]
-            if not last:
-                out.copy_from(Z)
-            L.check(lib.gtn_mlp_bias_act(L.ptr(out), self.E, n, b, L.ptr(self.b(l)),
+            L.check(lib.gtn_mlp_bias_act(L.ptr(A), self.E, n, b, L.ptr(self.b(l)),
                                          L.GTN_ACT_NONE if last else L.GTN_ACT_GELU,
-                                         None if last else L.ptr(Z), L.stream_ptr()), "gtn_mlp_bias_act")
-            H, hshared = out, False
+                                         None if last else L.ptr(self.P[l]), L.stream_ptr()), "gtn_mlp_bias_act")
+            H, hshared = A, False
         return H
 
     def backward(self, X, dout):
@@ -114,16 +114,16 @@ class MLPEnsemble:
         dZ = dout
         for l in range(self.L - 1, -1, -1):
             a, b = self.dims[l], self.dims[l + 1]
-            Hin, shared = (X, True) if l == 0 else (self.H[l - 1], False)
+            Hin, shared = (X, True) if l == 0 else (self.A[l - 1], False)
             # dW = H^T dZ (E, a, b); db = column sums of dZ
             self._mm(Hin, 1, a, 0 if shared else n * a, dZ, b, 1, n * b, self.W(l, self.grads), b, a, b, n, a * b)
-            L.check(lib.gtn_mlp_colsum(L.ptr(dZ), self.E, n, b, L.ptr(self.b(l, self.grads)), L.stream_ptr()),
-                    "gtn_mlp_colsum")
+            L.check(lib.gtn_mlp_colsum(L.ptr(dZ), self.E, n, b, L.ptr(self.b(l, self.grads)), L.ptr(self.colpart),
+                                       L.stream_ptr()), "gtn_mlp_colsum")
             if l > 0:
                 dH = self.dZ[l - 1]
                 # dH = dZ W^T (E, n, a), then through GELU' of the layer below's pre-activation
                 self._mm(dZ, b, 1, n * b, self.W(l), 1, b, a * b, dH, a, n, a, b, n * a)
-                L.check(lib.gtn_mlp_act_backward(L.ptr(dH), L.ptr(self.Z[l - 1]), self.E * n * a, L.GTN_ACT_GELU,
+                L.check(lib.gtn_mlp_act_backward(L.ptr(dH), L.ptr(self.P[l - 1]), self.E * n * a, L.GTN_ACT_GELU,
                                                  L.stream_ptr()), "gtn_mlp_act_backward")
                 dZ = dH
 

@@ -122,10 +122,11 @@ class CouplingSweep:
     def _coef(beta, m, idx):
         return np.stack([np.ones(len(idx)), beta[idx], m[idx]], axis=1)
 
-    def run(self, beta, m, chi, n_sv=0):
+    def run(self, beta, m, chi, n_sv=0, progress=None):
         """ln Z per site (complex where a final supertrace is negative) of every point; with n_sv > 0
         also the n_sv leading singular values of every point's final step (T1's blocks, decreasing,
-        divided by the largest; zero-padded), as (lnz, sv)."""
+        divided by the largest; zero-padded), as (lnz, sv).  progress(message), when given, is called
+        after every batch."""
         beta, m, chi = np.asarray(beta), np.asarray(m), np.asarray(chi)
         out = np.empty(len(beta), dtype=np.complex128)
         sv = np.zeros((len(beta), n_sv))
@@ -138,6 +139,8 @@ class CouplingSweep:
             if n_sv:
                 k = min(n_sv, run.final_sv.shape[1])
                 sv[idx, :k] = run.final_sv[:len(idx), :k]
+            if progress:
+                progress("chi %d: %d points done" % (c, len(idx)))
         return (out, sv) if n_sv else out
 
     def close(self):

@@ -295,7 +295,8 @@ int gtn_svd_blocks(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts
  * gtn_mlp_bias_act: y += bias (per column); pre (optional) = y; y = act(y) (GTN_ACT_GELU:
  *   x Phi(x) = 0.5 x (1 + erf(x / sqrt 2)); GTN_ACT_NONE: identity).
  * gtn_mlp_act_backward: dy *= GELU'(pre) over count values.
- * gtn_mlp_colsum: out[e][j] = sum_r x[e][r][j] (bias gradients; FP64 accumulation).
+ * gtn_mlp_colsum: out[e][j] = sum_r x[e][r][j] (bias gradients; FP64 accumulation in a fixed order:
+ *   partial sums of row chunks in part_dev, gtn_mlp_colsum_workspace(E, c) bytes, then their sum).
  * gtn_mlp_mse: with target (n x c) shared by the members and weight (c): dout = 2 w_j (out - t) / n,
  *   loss[e] = sum w_j (out - t)^2 / n (loss may be NULL).
  * gtn_adamw: AdamW (decoupled weight decay) on count parameters at step t = step[0] + 1, learning
@@ -311,7 +312,8 @@ typedef struct gtn_adamw_opts {
 int gtn_mlp_bias_act(float *y_dev, int64_t E, int64_t n, int64_t c, const float *bias_dev, int act, float *pre_dev,
                      void *stream);
 int gtn_mlp_act_backward(float *dy_dev, const float *pre_dev, int64_t count, int act, void *stream);
-int gtn_mlp_colsum(const float *x_dev, int64_t E, int64_t n, int64_t c, float *out_dev, void *stream);
+int gtn_mlp_colsum(const float *x_dev, int64_t E, int64_t n, int64_t c, float *out_dev, double *part_dev, void *stream);
+int64_t gtn_mlp_colsum_workspace(int64_t E, int64_t c); /* bytes of part_dev */
 int gtn_mlp_mse(const float *out_dev, const float *target_dev, const float *weight_dev, int64_t E, int64_t n, int64_t c,
                 float *dout_dev, double *loss_dev, void *stream);
 int gtn_adamw(float *w_dev, const float *g_dev, float *m_dev, float *v_dev, int64_t count, const gtn_adamw_opts *opts,

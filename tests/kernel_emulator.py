@@ -415,7 +415,10 @@ class FakeLib:
         d[:] = d * gelu_grad(self._arr(pre, count, np.float32, "act_backward pre").astype(np.float64))
         return 0
 
-    def gtn_mlp_colsum(self, x, E, n, c, out, stream):
+    def gtn_mlp_colsum_workspace(self, E, c):
+        return E * c * 32 * 8
+
+    def gtn_mlp_colsum(self, x, E, n, c, out, part, stream):
         X = self._arr(x, E * n * c, np.float32, "colsum x").reshape(E, n, c).astype(np.float64)
         self._arr(out, E * c, np.float32, "colsum out")[:] = X.sum(axis=1).reshape(-1)
         return 0
@@ -552,6 +555,7 @@ class FakeLib:
 
 
 _NOT_RECORDED = {"gtn_malloc", "gtn_free", "gtn_version", "gtn_last_error", "gtn_device_info", "gtn_set_dry_run",
+                 "gtn_mlp_colsum_workspace",
                  "gtn_svd_workspace_bytes", "gtn_gemm_noalloc_max_scratch", "gtn_event_create", "gtn_event_destroy",
                  "gtn_stream_create", "gtn_stream_destroy", "gtn_graph_begin", "gtn_graph_end", "gtn_graph_launch",
                  "gtn_graph_destroy", "gtn_pool_release"}
