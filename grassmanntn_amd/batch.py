@@ -128,7 +128,7 @@ def svd_batch(calls, cutoff):
             t += 1
     opts = L.SvdOpts()
     opts.oversample = settings.svd_oversample
-    opts.iters = settings.svd_iters
+    opts.iters = core.svd_iters_for(max(j["k"] for j in jobs))
     opts.precision = L.PREC[settings.svd_precision]
     opts.mode = 0
     opts.seed = settings.seed
