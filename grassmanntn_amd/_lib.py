@@ -81,6 +81,14 @@ class AdamwOpts(C.Structure):
     ]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [
+        ("dtype", C.c_int32), ("precision", C.c_int32), ("n_groups", C.c_int32), ("tile", C.c_int32),
+        ("split_target", C.c_int32), ("pad_", C.c_int32), ("M", C.c_int64), ("N", C.c_int64), ("K", C.c_int64),
+        ("batch", C.c_int64), ("ms_best", C.c_float), ("ms_default", C.c_float),
+    ]
+
+
 _lib = None
 _checked = False
 
@@ -109,6 +117,9 @@ def _bind(lib):
         "gtn_gemm_grouped_noalloc": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),
         "gtn_gemm_noalloc_max_scratch": (C.c_int64, []),
         "gtn_mfma_selftest": (C.c_int, [vp, vp]),
+        "gtn_autotune_set": (C.c_int, [i32]),
+        "gtn_autotune_clear": (C.c_int, []),
+        "gtn_autotune_report": (C.c_int, [C.POINTER(TuneEntry), i32]),
         "gtn_sumsq": (C.c_int, [vp, i64, i32, vp, vp]),
         "gtn_scale_by_norm": (C.c_int, [vp, i64, i32, vp, vp]),
         "gtn_reduce_last": (C.c_int, [vp, vp, i64, i64, i32, vp]),

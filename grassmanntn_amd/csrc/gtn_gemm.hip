@@ -526,11 +526,13 @@ static int scratch_ptr(void **out) {
 // The grouped GEMM.  fixed_ws == nullptr: split-reduction partials go to the library's per-device
 // workspace (allocated and grown here).  Otherwise they go to fixed_ws (fixed_bytes long) and a
 // split plan that needs more is an error; nothing is allocated.
+// tile <= 0 / target < 0: the heuristics (pick_tile, split_target)
 static int gemm_grouped_impl(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream,
-                             void *fixed_ws, int64_t fixed_bytes, const char *name) {
+                             void *fixed_ws, int64_t fixed_bytes, const char *name, int tile_cfg = 0,
+                             int target_cfg = -1) {
     GTN_REQUIRE(n_groups >= 0 && (n_groups == 0 || groups), "%s: bad groups", name);
     GTN_REQUIRE(dtype == GTN_F64 || dtype == GTN_F32, "gtn_gemm_grouped: dtype must be F64 or F32");
-    const int tile = pick_tile(groups, n_groups);
+    const int tile = tile_cfg > 0 ? tile_cfg : pick_tile(groups, n_groups);
     const int BM = tile, BN = tile;
     hipStream_t s = as_stream(stream);
     for (int base = 0; base < n_groups; base += kMaxGroups) {
@@ -606,7 +608,7 @@ static int gemm_grouped_impl(const gtn_gemm_desc *groups, int n_groups, int dtyp
         std::vector<GemmGroup> orig;
         std::unique_lock<std::mutex> lk(g_ws_mu, std::defer_lock);
         SplitWs *ws = nullptr;
-        const int target = split_target();
+        const int target = target_cfg >= 0 ? target_cfg : split_target();
         if (tiles < 512 && target > 0 && tiles * 2 <= target) {
             int S = 1;
             while (tiles * S < target && S < 16) S *= 2;
@@ -689,19 +691,189 @@ static int gemm_grouped_impl(const gtn_gemm_desc *groups, int n_groups, int dtyp
     return GTN_OK;
 }
 
+// ---- autotuner: launch configuration per call signature -------------------------------------
+// Candidates: tile 64 or 128 (128 only where every group is at least 128 x 128) times split-K
+// target 0 (no split), 512, 1024 or 2048 workgroups.  The first eager call of a signature (element
+// type, precision, and per group M, N, K, batch and operand layouts) times every candidate with HIP
+// events (best of kTuneReps launches each) on its own operands and keeps the fastest; calls made while
+// the stream is capturing a graph, and calls with beta != 0 (whose output a trial would change), use
+// the tuned configuration when there is one and the heuristics otherwise.
+struct TuneCfg {
+    int tile, target;
+    float ms, ms_default;
+};
+static std::mutex g_tune_mu;
+static std::map<std::string, TuneCfg> g_tuned;
+static int g_autotune = -1;
+constexpr int kTuneReps = 3;
+
+static bool autotune_on() {
+    if (g_autotune < 0) {
+        const char *e = getenv("GTN_AUTOTUNE");
+        g_autotune = e ? (atoi(e) != 0) : 1;
+    }
+    return g_autotune == 1;
+}
+
+static std::string tune_key(const gtn_gemm_desc *g, int n, int dtype, int prec, int dev) {
+    std::string k;
+    auto put = [&](int64_t v) { k.append((const char *)&v, sizeof(v)); };
+    put(dev);
+    put(dtype);
+    put(prec);
+    put(n);
+    for (int i = 0; i < n; ++i) {
+        put(g[i].M);
+        put(g[i].N);
+        put(g[i].K);
+        put(g[i].batch);
+        put((g[i].sak == 1) | ((g[i].sbk == 1) << 1) | ((g[i].scn == 1) << 2));
+    }
+    return k;
+}
+
+static int time_cfg(const gtn_gemm_desc *groups, int n, int dtype, int prec, hipStream_t s, int tile, int target,
+                    hipEvent_t e0, hipEvent_t e1, float *best) {
+    *best = 1e30f;
+    for (int r = 0; r < kTuneReps; ++r) {
+        GTN_CHECK_HIP(hipEventRecord(e0, s));
+        int rc = gemm_grouped_impl(groups, n, dtype, prec, s, nullptr, 0, "gtn_gemm_grouped", tile, target);
+        if (rc) return rc;
+        GTN_CHECK_HIP(hipEventRecord(e1, s));
+        GTN_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        GTN_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < *best) *best = ms;
+    }
+    return GTN_OK;
+}
+
+// the tuned configuration of this signature, if any (no timing)
+static int tuned_lookup(const gtn_gemm_desc *groups, int n, int dtype, int prec, int *tile, int *target, bool *found) {
+    *tile = 0;
+    *target = -1;
+    *found = false;
+    if (!autotune_on() || dry_run() || n <= 0) return GTN_OK;
+    int dev;
+    GTN_CHECK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    auto it = g_tuned.find(tune_key(groups, n, dtype, prec, dev));
+    if (it != g_tuned.end()) {
+        *tile = it->second.tile;
+        *target = it->second.target;
+        *found = true;
+    }
+    return GTN_OK;
+}
+
+static int tuned_config(const gtn_gemm_desc *groups, int n, int dtype, int prec, hipStream_t s, int *tile, int *target) {
+    bool found;
+    int rc0 = tuned_lookup(groups, n, dtype, prec, tile, target, &found);
+    if (rc0 || found || !autotune_on() || dry_run() || n <= 0) return rc0;
+    int dev;
+    GTN_CHECK_HIP(hipGetDevice(&dev));
+    const std::string key = tune_key(groups, n, dtype, prec, dev);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GTN_CHECK_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone || capture_open()) return GTN_OK;
+    bool can128 = true;
+    for (int i = 0; i < n; ++i) {
+        if (groups[i].beta != 0.0) return GTN_OK;
+        if (groups[i].M == 0 || groups[i].N == 0 || groups[i].batch == 0) continue;
+        if (groups[i].M < 128 || groups[i].N < 128) can128 = false;
+    }
+    hipEvent_t e0, e1;
+    GTN_CHECK_HIP(hipEventCreate(&e0));
+    GTN_CHECK_HIP(hipEventCreate(&e1));
+    TuneCfg best{0, -1, 1e30f, 0};
+    float ms_default = 0;
+    int rc = time_cfg(groups, n, dtype, prec, s, 0, -1, e0, e1, &ms_default);
+    const int tiles[2] = {64, 128};
+    const int targets[4] = {0, 512, 1024, 2048};
+    for (int ti = 0; ti < 2 && !rc; ++ti) {
+        if (tiles[ti] == 128 && !can128) continue;
+        for (int gi = 0; gi < 4 && !rc; ++gi) {
+            float ms;
+            rc = time_cfg(groups, n, dtype, prec, s, tiles[ti], targets[gi], e0, e1, &ms);
+            if (!rc && ms < best.ms) best = TuneCfg{tiles[ti], targets[gi], ms, 0};
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    best.ms_default = ms_default;
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    g_tuned[key] = best;
+    *tile = best.tile;
+    *target = best.target;
+    return GTN_OK;
+}
+
 extern "C" int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream) {
-    return gemm_grouped_impl(groups, n_groups, dtype, precision, stream, nullptr, 0, "gtn_gemm_grouped");
+    int tile = 0, target = -1;
+    int rc = tuned_config(groups, n_groups, dtype, precision, as_stream(stream), &tile, &target);
+    if (rc) return rc;
+    return gemm_grouped_impl(groups, n_groups, dtype, precision, stream, nullptr, 0, "gtn_gemm_grouped", tile, target);
+}
+
+extern "C" int gtn_autotune_set(int on) {
+    g_autotune = on ? 1 : 0;
+    return GTN_OK;
+}
+
+extern "C" int gtn_autotune_clear(void) {
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    g_tuned.clear();
+    return GTN_OK;
+}
+
+extern "C" int gtn_autotune_report(gtn_tune_entry *out, int max_entries) {
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    int i = 0;
+    for (auto &kv : g_tuned) {
+        if (!out || i >= max_entries) break;
+        const int64_t *k = (const int64_t *)kv.first.data();
+        gtn_tune_entry &e = out[i++];
+        memset(&e, 0, sizeof(e));
+        e.dtype = (int32_t)k[1];
+        e.precision = (int32_t)k[2];
+        e.n_groups = (int32_t)k[3];
+        e.M = k[4];
+        e.N = k[5];
+        e.K = k[6];
+        e.batch = k[7];
+        e.tile = kv.second.tile;
+        e.split_target = kv.second.target;
+        e.ms_best = kv.second.ms;
+        e.ms_default = kv.second.ms_default;
+    }
+    return (int)g_tuned.size();
 }
 
 extern "C" int64_t gtn_gemm_noalloc_max_scratch(void) { return kNoAllocScratchBytes; }
 
+// Follows gtn_gemm_grouped's tuned configuration of the signature (never timing one itself), unless
+// that configuration's split plan would not fit the scratch: then the heuristics, whose split plans
+// always fit (at most ~2 x 1024 workgroups of 64 x 64 FP64 partials = 64 MiB).
 extern "C" int gtn_gemm_grouped_noalloc(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision,
                                         void *stream) {
     void *p = nullptr;
     int rc = scratch_ptr(&p);
     if (rc) return rc;
+    int tile = 0, target = -1;
+    bool found = false;
+    rc = tuned_lookup(groups, n_groups, dtype, precision, &tile, &target, &found);
+    if (rc) return rc;
+    if (found && target > 1024) {
+        // a larger split target than the heuristics' may need more scratch than there is
+        rc = gemm_grouped_impl(groups, n_groups, dtype, precision, stream, p, kNoAllocScratchBytes,
+                               "gtn_gemm_grouped_noalloc", tile, target);
+        if (rc != GTN_ENOMEM) return rc;
+        tile = 0;
+        target = -1;
+    }
     return gemm_grouped_impl(groups, n_groups, dtype, precision, stream, p, kNoAllocScratchBytes,
-                             "gtn_gemm_grouped_noalloc");
+                             "gtn_gemm_grouped_noalloc", tile, target);
 }
 
 extern "C" int gtn_mfma_selftest(double *out_dev, void *stream) {

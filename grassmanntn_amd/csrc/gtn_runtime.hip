@@ -38,6 +38,22 @@ static std::map<int, Pool> g_pools;  // pool id (> 0) -> its free blocks
 static int g_cur_pool = 0;           // pool of the open capture, 0 outside captures
 static std::atomic<int> g_graphs{0};
 
+// Cached blocks beyond kCacheCap are released (hipFree) from the largest size classes first, never
+// from class keep_cls (the one being returned: it is the likeliest to be asked for again), and never
+// while a capture is open (hipFree synchronises the device).  Caller holds g_mu.
+static void trim_cache(int64_t keep_cls) {
+    if (g_cur_pool != 0) return;
+    while (g_cached_bytes > kCacheCap) {
+        auto victim = g_free.end();
+        for (auto it = g_free.begin(); it != g_free.end(); ++it)
+            if (it->first != keep_cls && !it->second.empty()) victim = it;  // map order: the last is the largest
+        if (victim == g_free.end()) break;
+        (void)hipFree(victim->second.back());
+        victim->second.pop_back();
+        g_cached_bytes -= victim->first;
+    }
+}
+
 int graphs_alive() { return g_graphs.load(); }
 int capture_open() {
     std::lock_guard<std::mutex> l(g_mu);
@@ -119,13 +135,10 @@ int gtn_free(void *p) {
             return GTN_OK;
         }
     }
-    // the general cache; never hipFree while a capture is open or a graph may hold the block
-    if (g_cached_bytes + cls > kCacheCap && g_cur_pool == 0 && g_graphs.load() == 0) {
-        GTN_CHECK_HIP(hipFree(p));
-        return GTN_OK;
-    }
+    // the general cache (graphs hold pool blocks only); over the cap, other size classes make room
     g_free[cls].push_back(p);
     g_cached_bytes += cls;
+    if (g_cached_bytes > kCacheCap) trim_cache(cls);
     return GTN_OK;
 }
 
@@ -282,6 +295,7 @@ int gtn_pool_release(int pool) {
         }
     P.free.clear();
     P.released = true;
+    trim_cache(-1);
     return GTN_OK;
 }
 

@@ -188,10 +188,33 @@ typedef struct gtn_gemm_desc {
 } gtn_gemm_desc;
 int gtn_gemm_grouped(const gtn_gemm_desc *groups, int n_groups, int dtype, int precision, void *stream);
 
+/* Autotuner of gtn_gemm_grouped's launch configuration (no counterpart in the reference).  With it on
+ * (the default; GTN_AUTOTUNE=0 in the environment or gtn_autotune_set(0) turns it off) the first
+ * eager call of each signature -- element type, precision, and per group M, N, K, batch and operand
+ * layouts -- runs every candidate configuration (tile 64 or 128, the latter only where every group
+ * is at least 128 x 128; split-K target off, 512, 1024 or 2048 workgroups) on its own operands, times
+ * each with HIP events (best of 3), and keeps the fastest for every later call of that signature in
+ * the process.  Calls issued while their stream is capturing a graph, and calls with beta != 0, are
+ * never timed: they use the tuned configuration if there is one, else the heuristics (128 x 128 tiles
+ * when every group is at least 128 x 128 and the launch has >= 256 of them; split-K towards 1024
+ * workgroups when it has < 512 tiles).  Results are identical up to the summation order of split-K
+ * partials.  gtn_autotune_report copies up to max_entries tuned signatures (first group's sizes) and
+ * returns how many there are. */
+typedef struct gtn_tune_entry {
+    int32_t dtype, precision, n_groups, tile, split_target, pad_;
+    int64_t M, N, K, batch;
+    float ms_best, ms_default;
+} gtn_tune_entry;
+int gtn_autotune_set(int on);
+int gtn_autotune_clear(void);
+int gtn_autotune_report(gtn_tune_entry *out, int max_entries);
+
 /* Allocation-free variant of gtn_gemm_grouped.  It has no counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, kernels, split plan and results (bit for bit)
- * as gtn_gemm_grouped; the split-reduction partials that gtn_gemm_grouped keeps in a workspace it
+ * as gtn_gemm_grouped (it follows the autotuner's configuration of the signature, below, when that
+ * one's split plan fits the scratch, else the default plan); the split-reduction partials that
+ * gtn_gemm_grouped keeps in a workspace it
  * allocates go instead to a fixed per-device scratch array of gtn_gemm_noalloc_max_scratch() bytes
  * (64 MiB) that the library holds from its first use on a device until the process ends.  Nothing
  * is allocated or freed here.  A call whose split plan needs more scratch than that fails with

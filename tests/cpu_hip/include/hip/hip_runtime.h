@@ -218,6 +218,11 @@ typedef void *hipGraphExec_t;
 typedef void *hipGraphNode_t;
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
 #define hipErrorNotSupported 801
+enum hipStreamCaptureStatus { hipStreamCaptureStatusNone = 0, hipStreamCaptureStatusActive, hipStreamCaptureStatusInvalidated };
+inline hipError_t hipStreamIsCapturing(hipStream_t, hipStreamCaptureStatus *st) {
+    *st = hipStreamCaptureStatusNone;
+    return hipSuccess;
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode m);

@@ -460,6 +460,15 @@ class FakeLib:
     def gtn_gemm_grouped_noalloc(self, groups, n, dtype, prec, stream):
         return self.gtn_gemm_grouped(groups, n, dtype, prec, stream)
 
+    def gtn_autotune_set(self, on):
+        return 0
+
+    def gtn_autotune_clear(self):
+        return 0
+
+    def gtn_autotune_report(self, out, n):
+        return 0
+
     def gtn_gemm_noalloc_max_scratch(self):
         return 64 << 20
 
@@ -555,7 +564,7 @@ class FakeLib:
 
 
 _NOT_RECORDED = {"gtn_malloc", "gtn_free", "gtn_version", "gtn_last_error", "gtn_device_info", "gtn_set_dry_run",
-                 "gtn_mlp_colsum_workspace",
+                 "gtn_mlp_colsum_workspace", "gtn_autotune_set", "gtn_autotune_clear", "gtn_autotune_report",
                  "gtn_svd_workspace_bytes", "gtn_gemm_noalloc_max_scratch", "gtn_event_create", "gtn_event_destroy",
                  "gtn_stream_create", "gtn_stream_destroy", "gtn_graph_begin", "gtn_graph_end", "gtn_graph_launch",
                  "gtn_graph_destroy", "gtn_pool_release"}
