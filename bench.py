@@ -5,9 +5,11 @@ and the learned surrogate.
 Headline (value): TRG steps/s at leg dim 64, chi 64 of the mixed-precision batched path -- P seeded
 Grassmann-even site tensors (entries N(0,1) times exp(-0.5 * sum of leg popcounts), odd total parity
 zeroed, max|T| = 1, parity-preserving legs, statistics (1,1,-1,-1); seeds 1..P) coarse-grained
-together for --steps (20) steps, every step one replay of a recorded HIP graph; arithmetic: the first
-five steps FP64, then contractions in FP32 MFMA with FP32 accumulation, while the parity-block SVD
-(range finder included), the normalisation and the log-norm sum stay FP64 at every step.  Steps/s
+together for --steps (20) steps, every step one replay of a recorded HIP graph; arithmetic
+(gtn.mixed_schedule): contractions FP64 in the first five steps, then FP32 MFMA with FP32
+accumulation; in every step the parity-block SVD's range finder runs in split BF16 (three BF16
+products, FP32 accumulation) except its last three iterations, which are FP64, while the
+Rayleigh-Ritz step, the singular vectors, the normalisation and the log-norm sum are FP64.  Steps/s
 counts (point, step) pairs.  Beside it: the plain FP64 GPU path (one tensor, the eager gtn.trg loop)
 and the FP64 batched path (same batch, all FP64), ln Z per site of every point with an oracle golden
 against that golden, the roofline of the step's dominant kernel (the range finder's FP64 GEMM, timed
@@ -348,7 +350,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="f32", help="contraction arithmetic of the mixed path")
-    ap.add_argument("--svd-precision", default="f64", help="range-finder arithmetic of the mixed path")
+    ap.add_argument("--svd-precision", default=None, help="range-finder arithmetic of the mixed path "
+                    "(default: the library's mixed schedule, split BF16 with three FP64 iterations at the end)")
     ap.add_argument("--fp64-steps", type=int, default=5,
                     help="leading TRG steps of the mixed path run in FP64 (their normalisation errors "
                          "weigh 2^-n in ln Z per site)")
@@ -385,7 +388,8 @@ def main():
         gtn.settings.svd_oversample = args.svd_oversample
     P, K = args.batch, args.steps
     prec = args.precision
-    mixed_sched = lambda i: ("f64", "f64") if i < args.fp64_steps else (prec, args.svd_precision)
+    svd_prec = args.svd_precision or gtn.core.MIXED_SVD
+    mixed_sched = lambda i: gtn.mixed_schedule(i, prec, svd=svd_prec, fp64_steps=args.fp64_steps)
     T0s = np.stack([site_tensor(D, SEED + s) for s in range(P)])
     golden = golden_lnz(K)
     progress("mixed batched path: %d points, %d steps" % (P, K))
@@ -403,8 +407,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 for steps 1-%d, then %s contractions; f64 SVD, normalisation and log-norm sum" % (
-            args.fp64_steps, prec),
+        "dtype": "f64 contractions in steps 1-%d, then %s; range finder %s; f64 Rayleigh-Ritz, normalisation "
+                 "and log-norm sum" % (args.fp64_steps, prec, mixed_sched(0)[1]),
         "data": "synthetic",
         "config": {"workload": "TRG at leg dim 64, chi 64: %d seeded site tensors (seeds %d..%d) coarse-grained "
                                "together for %d steps, one HIP-graph replay per step" % (P, SEED, SEED + P - 1, K),

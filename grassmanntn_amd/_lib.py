@@ -70,7 +70,7 @@ class SvdBlock(C.Structure):
 class SvdOpts(C.Structure):
     _fields_ = [
         ("oversample", C.c_int32), ("iters", C.c_int32), ("precision", C.c_int32), ("mode", C.c_int32),
-        ("seed", C.c_uint64), ("orth_every", C.c_int32), ("pad_", C.c_int32),
+        ("seed", C.c_uint64), ("orth_every", C.c_int32), ("f64_tail", C.c_int32),
     ]
 
 
@@ -136,7 +136,7 @@ def _bind(lib):
         "gtn_graph_launch": (C.c_int, [vp, vp]),
         "gtn_graph_destroy": (C.c_int, [vp]),
         "gtn_pool_release": (C.c_int, [i32]),
-        "gtn_normalize_batch": (C.c_int, [vp, i64, i64, i32, vp, vp, vp, i32, vp, vp]),
+        "gtn_normalize_batch": (C.c_int, [vp, vp, i64, i64, i32, vp, vp, vp, i32, vp, vp]),
         "gtn_diag_interleave": (C.c_int, [vp, i64, i64, vp, i32, vp]),
         "gtn_mlp_bias_act": (C.c_int, [vp, i64, i64, i64, vp, i32, vp, vp]),
         "gtn_mlp_act_backward": (C.c_int, [vp, vp, i64, i32, vp]),

@@ -129,7 +129,7 @@ def svd_batch(calls, cutoff):
     opts = L.SvdOpts()
     opts.oversample = settings.svd_oversample
     opts.iters = core.svd_iters_for(max(j["k"] for j in jobs))
-    opts.precision = L.PREC[settings.svd_precision]
+    opts.precision, opts.f64_tail = core.svd_spec(settings.svd_precision)
     opts.mode = 0
     opts.seed = settings.seed
     opts.orth_every = settings.svd_orth_every
@@ -186,13 +186,17 @@ def trg_batch_step(Tb, dcut):
     return einsum("Blxzk,Bjzxi->Bijkl", VV, UU), sv
 
 
-def normalize_batch(T, nrm2, lognorm=None, hist=None, step=None):
-    """In place: every point of T[B, ...] divided by its Frobenius norm; nrm2[b] gets the squared
-    norm, lognorm[b] the driver's 2 lognorm[b] + log ||T_b||, hist[b, step] = log ||T_b||."""
+def normalize_batch(T, nrm2, lognorm=None, hist=None, step=None, out=None):
+    """Every point of T[B, ...] divided by its Frobenius norm, in place or into the device array
+    `out` (T's element count); nrm2[b] gets the squared norm, lognorm[b] the driver's
+    2 lognorm[b] + log ||T_b||, hist[b, step] = log ||T_b||."""
     P = T.shape[0]
     n = T.data.numel() // P
     hl = hist.shape[1] if hist is not None else 0
-    L.check(L.lib().gtn_normalize_batch(L.ptr(T.data), P, n, L.dtype_code(T.data), L.ptr(nrm2),
+    if out is not None and out.numel() != T.data.numel():
+        raise ValueError("normalize_batch: output of %d values for %d" % (out.numel(), T.data.numel()))
+    L.check(L.lib().gtn_normalize_batch(L.ptr(T.data), L.ptr(out) if out is not None else None, P, n,
+                                        L.dtype_code(T.data), L.ptr(nrm2),
                                         L.ptr(lognorm) if lognorm is not None else None,
                                         L.ptr(hist) if hist is not None else None, hl,
                                         L.ptr(step) if step is not None else None, L.stream_ptr()),
@@ -287,14 +291,10 @@ class BatchedTRG:
         with precision(*prec):
             Tn, sv = trg_batch_step(T, self.dcut)
         if bookkeeping:
-            normalize_batch(Tn, self.nrm2, self.lognorm, self.hist, self.stepc)
+            normalize_batch(Tn, self.nrm2, self.lognorm, self.hist, self.stepc, out=x_out)
             copy2d(self.svals, self.Dn, sv.reshape((self.P, self.Dn)), self.Dn, self.P, self.Dn)
         else:
-            normalize_batch(Tn, DeviceArray.empty(self.P))
-        src = Tn.data
-        cols = 4096 if src.numel() % (4096 * self.P) == 0 else src.numel() // self.P
-        rows = src.numel() // cols
-        copy2d(x_out.reshape((rows, cols)), cols, src.reshape((rows, cols)), cols, rows, cols)
+            normalize_batch(Tn, DeviceArray.empty(self.P), out=x_out)
 
     def _key(self, i):
         return (0 if i == 0 else 1, tuple(self.schedule(i)))

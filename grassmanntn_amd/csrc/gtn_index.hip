@@ -241,14 +241,32 @@ __device__ __forceinline__ int64_t map_elem(const PermArgs &a, const TabArgs &t,
         return perm_map(a, lin, code);
 }
 
+// kPermUnroll output elements per lane and pass (a grid stride apart, so each pass stays coalesced):
+// their index maps and table reads are issued together, then their source reads, so a lane keeps
+// kPermUnroll dependent load chains in flight instead of one.
+constexpr int kPermUnroll = 4;
 template <typename T, bool TAB>
-__global__ void k_permute(PermArgs a, TabArgs tb, const T *__restrict__ src, T *__restrict__ dst) {
+__global__ void __launch_bounds__(256) k_permute(PermArgs a, TabArgs tb, const T *__restrict__ src, T *__restrict__ dst) {
     const double al = a.alpha_re;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.total; i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t code;
-        int64_t off = map_elem<TAB>(a, tb, (uint64_t)i, code);
-        double v = (code == 2u) ? 0.0 : (double)src[off] * (code ? -al : al);
-        dst[i] = (T)v;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < a.total; i0 += kPermUnroll * stride) {
+        int64_t off[kPermUnroll];
+        uint32_t code[kPermUnroll];
+#pragma unroll
+        for (int u = 0; u < kPermUnroll; ++u) {
+            const int64_t i = i0 + u * stride;
+            code[u] = 2u;
+            off[u] = 0;
+            if (i < a.total) off[u] = map_elem<TAB>(a, tb, (uint64_t)i, code[u]);
+        }
+        double v[kPermUnroll];
+#pragma unroll
+        for (int u = 0; u < kPermUnroll; ++u) v[u] = (code[u] == 2u) ? 0.0 : (double)src[off[u]];
+#pragma unroll
+        for (int u = 0; u < kPermUnroll; ++u) {
+            const int64_t i = i0 + u * stride;
+            if (i < a.total) dst[i] = (T)(code[u] == 2u ? 0.0 : v[u] * (code[u] ? -al : al));
+        }
     }
 }
 
@@ -298,16 +316,28 @@ __global__ void __launch_bounds__(256) k_permute_tiled(PermArgs a, TabArgs tb, T
             base += (rr % d) * t.ostr[ax];
             rr /= d;
         }
-        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-            const int l1 = e & 63, l0 = e >> 6;
-            const int i1 = i1b + l1, i0 = i0b + l0;
-            double v = 0.0;
-            if (i1 < d1 && i0 < d0) {
-                uint32_t code;
-                const int64_t off = map_elem<TAB>(a, tb, (uint64_t)(base + i1 * s1 + i0), code);
-                v = (code == 2u) ? 0.0 : (double)src[off] * (code ? -al : al);
+        // 16 elements per lane in two batches of 8: index maps and table reads first, then the
+        // source reads, so that 8 load chains are in flight per lane
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int64_t off[8];
+            uint32_t code[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = threadIdx.x + 256 * (8 * h + u);
+                const int i1 = i1b + (e & 63), i0 = i0b + (e >> 6);
+                code[u] = 2u;
+                off[u] = 0;
+                if (i1 < d1 && i0 < d0) off[u] = map_elem<TAB>(a, tb, (uint64_t)(base + i1 * s1 + i0), code[u]);
             }
-            tile[l1][l0] = v;
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = (code[u] == 2u) ? 0.0 : (double)src[off[u]] * (code[u] ? -al : al);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = threadIdx.x + 256 * (8 * h + u);
+                tile[e & 63][e >> 6] = v[u];
+            }
         }
         __syncthreads();
         for (int e = threadIdx.x; e < 64 * 64; e += 256) {
@@ -556,11 +586,12 @@ __global__ void k_norm_batch_final(const double *part, int G, double *nrm2, doub
 }
 // pass 3: x_b /= ||x_b||, grid (G, nb)
 template <typename T>
-__global__ void k_scale_batch(T *x, int64_t n, const double *nrm2) {
+__global__ void k_scale_batch(const T *x, T *dst, int64_t n, const double *nrm2) {
     const double inv = 1.0 / sqrt(nrm2[blockIdx.y]);
-    T *xb = x + (int64_t)blockIdx.y * n;
+    const T *xb = x + (int64_t)blockIdx.y * n;
+    T *db = dst + (int64_t)blockIdx.y * n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        xb[i] = (T)((double)xb[i] * inv);
+        db[i] = (T)((double)xb[i] * inv);
 }
 __global__ void k_step_increment(double *step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) step[0] += 1.0;
@@ -954,7 +985,7 @@ int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream
         GTN_CHECK_LAUNCH();
         return GTN_OK;
     }
-    dim3 g(grid_for(a.total, 256, 1 << 20)), b(256);
+    dim3 g(grid_for((a.total + kPermUnroll - 1) / kPermUnroll, 256, 1 << 20)), b(256);
     switch (d->dtype) {
         case GTN_F64:
             if (tab)
@@ -1087,8 +1118,8 @@ int gtn_axpby(const void *x, const void *y, void *out, int64_t n, int dtype, dou
     return GTN_OK;
 }
 
-int gtn_normalize_batch(void *x, int64_t nbatch, int64_t n, int dtype, double *nrm2, double *lognorm, double *hist,
-                        int hist_len, double *step, void *stream) {
+int gtn_normalize_batch(void *x, void *dst, int64_t nbatch, int64_t n, int dtype, double *nrm2, double *lognorm,
+                        double *hist, int hist_len, double *step, void *stream) {
     GTN_REQUIRE(nbatch >= 1 && nbatch <= 1024 && n >= 1, "gtn_normalize_batch: need 1 <= nbatch <= 1024 and n >= 1");
     GTN_REQUIRE(x && nrm2, "gtn_normalize_batch: null buffer");
     GTN_REQUIRE(dtype == GTN_F64 || dtype == GTN_F32, "gtn_normalize_batch: dtype must be F64 or F32");
@@ -1107,10 +1138,11 @@ int gtn_normalize_batch(void *x, int64_t nbatch, int64_t n, int dtype, double *n
                        lognorm, hist, hist_len, (const double *)step);
     GTN_CHECK_LAUNCH();
     const dim3 g3((unsigned)grid_for(n, 256, (int)(65536 / nbatch) > 0 ? (int)(65536 / nbatch) : 1), (unsigned)nbatch);
+    if (!dst) dst = x;
     if (dtype == GTN_F64)
-        hipLaunchKernelGGL(k_scale_batch<double>, g3, b, 0, s, (double *)x, n, (const double *)nrm2);
+        hipLaunchKernelGGL(k_scale_batch<double>, g3, b, 0, s, (const double *)x, (double *)dst, n, (const double *)nrm2);
     else
-        hipLaunchKernelGGL(k_scale_batch<float>, g3, b, 0, s, (float *)x, n, (const double *)nrm2);
+        hipLaunchKernelGGL(k_scale_batch<float>, g3, b, 0, s, (const float *)x, (float *)dst, n, (const double *)nrm2);
     GTN_CHECK_LAUNCH();
     if (step) {
         hipLaunchKernelGGL(k_step_increment, dim3(1), dim3(64), 0, s, step);

@@ -15,6 +15,7 @@
 #include <string.h>
 #include <math.h>
 #include <vector>
+#include <utility>
 
 namespace gtn {
 
@@ -682,14 +683,187 @@ __global__ void k_gauge(double *U, int64_t m, int64_t ldu, double *Vt, int64_t n
 // ---------------------------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------
+// Range finder in split BF16.  Reduced-precision iterations keep their operands as BF16 "planes":
+// x = hi + lo + O(2^-16 |x|) with hi = bf16(fp32(x)), lo = bf16(fp32(x - hi)), each operand a pair of
+// row-major matrices (hi, then lo) whose rows run along the reduction index, zero-padded to 128 rows
+// and 128 columns.  The block M is converted once per call into planes of M and of M^T, the basis Y
+// into planes of Y^T every iteration, and Z = M^T Y leaves the GEMM as planes of Z^T directly, so
+// every product is C = A B^T over two K-contiguous operands (the layout the 16x16x32 BF16 MFMA reads
+// with one 16-byte load per lane) and the block's 16 bytes per element of FP64 become 4 (M) per pass.
+// ---------------------------------------------------------------------------------------------
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kRfTile = 128;  // rows per workgroup tile and the padded column count of the B planes
+constexpr int kRfBK = 64;     // reduction depth per LDS stage
+static inline int64_t rf_pad(int64_t x) { return (x + kRfTile - 1) / kRfTile * kRfTile; }
+
+__device__ __forceinline__ void split_bf16(double x, __bf16 &h, __bf16 &l) {
+    h = (__bf16)(float)x;
+    l = (__bf16)(float)(x - (double)(float)h);
+}
+
+// Planes of X (R x Cc, ld ldx; zero beyond) over the padded Rp x Cp grid: direct planes D (Rp x Cp,
+// hi at D, lo at D + Rp Cp) and / or transposed planes T (Cp x Rp, hi at T, lo at T + Rp Cp).
+// grid (Cp / 64, Rp / 64, blocks); blockIdx.z advances X by dX doubles and D, T by dP bf16 elements.
+__global__ void __launch_bounds__(256) k_planes(const double *X, int64_t ldx, int64_t R, int64_t Cc, int64_t Rp,
+                                                int64_t Cp, __bf16 *D, __bf16 *T, int64_t dX, int64_t dP) {
+    __shared__ double tile[64][65];
+    X += blockIdx.z * dX;
+    const int64_t plane = Rp * Cp;
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int tc = threadIdx.x & 63, tr = threadIdx.x >> 6;
+#pragma unroll 4
+    for (int i = tr; i < 64; i += 4) {
+        const int64_t r = r0 + i, c = c0 + tc;
+        const double x = (r < R && c < Cc) ? X[r * ldx + c] : 0.0;
+        tile[i][tc] = x;
+        if (D) {
+            __bf16 h, l;
+            split_bf16(x, h, l);
+            __bf16 *d = D + blockIdx.z * dP + r * Cp + c;
+            d[0] = h;
+            d[plane] = l;
+        }
+    }
+    if (!T) return;
+    __syncthreads();
+#pragma unroll 4
+    for (int i = tr; i < 64; i += 4) {
+        // T[c0 + i][r0 + tc] = X[r0 + tc][c0 + i]
+        __bf16 h, l;
+        split_bf16(tile[tc][i], h, l);
+        __bf16 *t = T + blockIdx.z * dP + (c0 + i) * Rp + r0 + tc;
+        t[0] = h;
+        t[plane] = l;
+    }
+}
+
+struct RfArgs {
+    const __bf16 *A, *B;     // hi planes; lo planes follow at +plane_a / +plane_b
+    void *C;
+    int64_t K;               // reduction length (a multiple of kRfBK; both operands zero beyond the data)
+    int64_t plane_a, plane_b;
+    int64_t rows, rows_pad;  // rows of C with data / rows of the A planes (a multiple of kRfTile)
+    int64_t cols, ldc;       // columns of C with data (<= kRfTile); FP64 output leading dimension
+    int64_t dA, dB, dC;      // per-block strides (bf16 elements for planes, doubles for FP64 C)
+    int32_t nb;              // blocks
+};
+
+// C = A B^T for every block, A (rows_pad x K) and B (kRfTile x K) in split-BF16 planes: the products
+// hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  OUT 0: C FP64
+// (rows x cols, ld ldc); OUT 1: C^T as planes (kRfTile x rows_pad, ld rows_pad), padding included
+// (zero where A or B is zero padding).  One workgroup (4 waves, 2 x 2, 64 x 64 each) per 128-row
+// tile; tiles of one block are enumerated contiguously per XCD so that they share its L2 (B).
+template <int OUT>
+__global__ void __launch_bounds__(256) k_rf_nt(RfArgs a) {
+    const int64_t tiles = a.rows_pad / kRfTile, total = tiles * a.nb;
+    const int64_t per = (total + 7) / 8;
+    const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= total) return;
+    const int64_t blk = t / tiles, m0 = (t - blk * tiles) * kRfTile;
+    const __bf16 *A = a.A + blk * a.dA + m0 * a.K;
+    const __bf16 *B = a.B + blk * a.dB;
+    constexpr int PAD = 8, LD = kRfBK + PAD;
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][kRfTile][LD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][kRfTile][LD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f4v acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0, 0, 0, 0};
+    // 16-byte chunk s of this thread: plane s / 4, row (tid + 256 (s % 4)) / 8, k chunk tid % 8
+    u32x4 ra[8], rb[8];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int c = threadIdx.x + 256 * (s & 3);
+            const int row = c >> 3, kc = c & 7;
+            const int64_t off = (int64_t)row * a.K + k0 + 8 * kc;
+            ra[s] = *(const u32x4 *)(A + (s >> 2) * a.plane_a + off);
+            rb[s] = *(const u32x4 *)(B + (s >> 2) * a.plane_b + off);
+        }
+    };
+    load(0);
+    for (int64_t k0 = 0; k0 < a.K; k0 += kRfBK) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int c = threadIdx.x + 256 * (s & 3);
+            const int row = c >> 3, kc = c & 7;
+            *(u32x4 *)&As[s >> 2][row][8 * kc] = ra[s];
+            *(u32x4 *)&Bs[s >> 2][row][8 * kc] = rb[s];
+        }
+        __syncthreads();
+        if (k0 + kRfBK < a.K) load(k0 + kRfBK);
+#pragma unroll
+        for (int kk = 0; kk < kRfBK; kk += 32) {
+            bf8v ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                ah[i] = *(const bf8v *)&As[0][wm + 16 * i + lane % 16][kk + 8 * (lane / 16)];
+                al[i] = *(const bf8v *)&As[1][wm + 16 * i + lane % 16][kk + 8 * (lane / 16)];
+                bh[i] = *(const bf8v *)&Bs[0][wn + 16 * i + lane % 16][kk + 8 * (lane / 16)];
+                bl[i] = *(const bf8v *)&Bs[1][wn + 16 * i + lane % 16][kk + 8 * (lane / 16)];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+    // result register v of lane l holds C[4 (l / 16) + v][l % 16] of each 16 x 16 tile
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t r = m0 + wm + 16 * i + 4 * (lane / 16);
+            const int64_t c = wn + 16 * j + lane % 16;
+            if (OUT == 0) {
+                double *C = (double *)a.C + blk * a.dC;
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (r + v < a.rows && c < a.cols) C[(r + v) * a.ldc + c] = (double)acc[i][j][v];
+            } else {
+                __bf16 *C = (__bf16 *)a.C + blk * a.dC + c * a.rows_pad + r;
+                __bf16 h[4], l[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) split_bf16((double)acc[i][j][v], h[v], l[v]);
+                u32x2 hv, lv;
+                memcpy(&hv, h, 8);
+                memcpy(&lv, l, 8);
+                *(u32x2 *)C = hv;
+                *(u32x2 *)(C + (int64_t)kRfTile * a.rows_pad) = lv;
+            }
+        }
+}
+
 struct Blk {
     gtn_svd_block b;
     int64_t p;
     bool small, left;
     double *ws;
     double *Y, *Z, *T, *Bm, *W, *G, *R, *V, *Vs, *lam, *inv, *keep, *X, *part, *Lg;
+    // split-BF16 planes of the range finder (planes_of(opts) and a large block): M (mp x np), M^T
+    // (np x mp), Y^T (kRfTile x mp), Z^T (kRfTile x np), each hi then lo
+    __bf16 *Mp, *MTp, *YTp, *ZTp;
+    int64_t mp, np;
     int64_t bytes;
 };
+
+// reduced range-finder iterations on split-BF16 planes (SVD mode, BF16X3 arithmetic)
+static bool planes_of(const gtn_svd_opts &o) {
+    return o.mode == 0 && o.precision == GTN_PREC_BF16X3 && o.iters > o.f64_tail;
+}
 
 static int pick_split(int64_t K) {
     int S = 1;
@@ -726,6 +900,12 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
     int64_t ol = take(p), oi = take(p), ok = take(p), oX = take(big * kk);
     int64_t opart = take((int64_t)32 * p * p);  // up to 32 split-K partial Gram matrices
     int64_t olg = take((int64_t)kMaxSweeps * (p - 1) * (p + 1));  // Jacobi rotation log: (c, s) per pair, round ids
+    P.mp = rf_pad(b.m);
+    P.np = rf_pad(b.n);
+    const bool planes = !P.small && planes_of(o);
+    // bf16 planes, in doubles: 2 (hi, lo) x 2 (M, M^T) x mp np / 4, 2 x kRfTile x (mp or np) / 4
+    int64_t oMp = planes ? take(P.mp * P.np) : 0, oYT = planes ? take(kRfTile * P.mp / 2) : 0,
+            oZT = planes ? take(kRfTile * P.np / 2) : 0;
     P.bytes = (off * 8 + 255) / 256 * 256;
     P.ws = ws;
     if (ws) {
@@ -744,6 +924,10 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
         P.X = ws + oX;
         P.part = ws + opart;
         P.Lg = ws + olg;
+        P.Mp = planes ? (__bf16 *)(ws + oMp) : nullptr;
+        P.MTp = planes ? P.Mp + 2 * P.mp * P.np : nullptr;
+        P.YTp = planes ? (__bf16 *)(ws + oYT) : nullptr;
+        P.ZTp = planes ? (__bf16 *)(ws + oZT) : nullptr;
     }
 }
 
@@ -784,6 +968,7 @@ struct Run {
     int n;           // blocks in the run (the batch of every launch)
     int first;       // index of the first block in the call (seeds)
     int64_t dM, dU, dS, dV, dws;
+    std::vector<Blk *> mem;  // every block of the run, f first
 };
 
 static bool delta_of(const void *a, const void *b, int64_t &d) {
@@ -817,10 +1002,12 @@ static std::vector<Run> make_runs(std::vector<Blk *> &bs, std::vector<int> &inde
                 if (r.n == 1) {
                     r.dM = dM, r.dU = dU, r.dS = dS, r.dV = dV, r.dws = dws;
                     ++r.n;
+                    r.mem.push_back(b);
                     continue;
                 }
                 if (dM == r.dM && dU == r.dU && dS == r.dS && dV == r.dV && dws == r.dws) {
                     ++r.n;
+                    r.mem.push_back(b);
                     continue;
                 }
             }
@@ -830,6 +1017,7 @@ static std::vector<Run> make_runs(std::vector<Blk *> &bs, std::vector<int> &inde
         r.n = 1;
         r.first = index[i];
         r.dM = r.dU = r.dS = r.dV = r.dws = 0;
+        r.mem.push_back(b);
         runs.push_back(r);
     }
     return runs;
@@ -988,13 +1176,10 @@ static int orthonormalise(std::vector<Run> &runs, int passes, hipStream_t s) {
         }
         rc = run(g, GTN_PREC_F64, s);
         if (rc) return rc;
-        for (Run &r : runs) {
-            Blk &f = *r.f;
-            int64_t rows = f.b.m, p = f.p;
-            R8N(f.T, rows * p, r.dws, r.n, "orth scratch");
-            R8N(f.Y, rows * p, r.dws, r.n, "orth basis");
-            LAUNCH(k_copy2d, grid2(rows * p, r.n), dim3(256), 0, s, (const double *)f.T, p, f.Y, p, rows, p, r.dws, r.dws);
-        }
+        // the product is the new basis: Y and its scratch trade places in every block (same offsets
+        // in every block's workspace, so the runs' strides hold)
+        for (Run &r : runs)
+            for (Blk *b : r.mem) std::swap(b->Y, b->T);
     }
     return GTN_OK;
 }
@@ -1035,6 +1220,62 @@ static int jacobi(std::vector<Run> &runs, int by_abs, hipStream_t s) {
         LAUNCH(k_jacobi_vectors, dim3((unsigned)p, r.n), dim3(64), 0, s, sb);
     }
     return GTN_OK;
+}
+
+// split-BF16 planes of every block's M and M^T (once per call)
+static int rf_planes_of_blocks(std::vector<Run> &rl, hipStream_t s) {
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        GTN_REQUIRE(f.Mp && f.mp / 64 <= 65535 && f.np / 64 <= 65535, "gtn_svd_blocks: no range-finder planes");
+        R8N(f.b.M, ext(f.b.m, f.b.n, f.b.lda), r.dM, r.n, "block");
+        R8N((const double *)f.Mp, f.mp * f.np, r.dws, r.n, "range-finder planes of M");
+        LAUNCH(k_planes, dim3((unsigned)(f.np / 64), (unsigned)(f.mp / 64), r.n), dim3(256), 0, s, f.b.M, f.b.lda, f.b.m,
+               f.b.n, f.mp, f.np, f.Mp, f.MTp, r.dM, 4 * r.dws);
+    }
+    return GTN_OK;
+}
+
+static dim3 rf_grid(int64_t rows_pad, int nb) {
+    const int64_t total = rows_pad / kRfTile * nb;
+    return dim3((unsigned)((total + 7) / 8 * 8));
+}
+
+// Y <- M Z (FP64) from the planes of M and of Z^T
+static int rf_y_from_zt(std::vector<Run> &rl, hipStream_t s) {
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        RfArgs a;
+        a.A = f.Mp, a.B = f.ZTp, a.C = f.Y;
+        a.K = f.np, a.plane_a = f.mp * f.np, a.plane_b = kRfTile * f.np;
+        a.rows = f.b.m, a.rows_pad = f.mp, a.cols = f.p, a.ldc = f.p;
+        a.dA = a.dB = 4 * r.dws, a.dC = r.dws, a.nb = r.n;
+        R8N(f.Y, f.b.m * f.p, r.dws, r.n, "range-finder basis");
+        LAUNCH(k_rf_nt<0>, rf_grid(f.mp, r.n), dim3(256), 0, s, a);
+    }
+    return GTN_OK;
+}
+
+// one reduced iteration on planes: Y^T planes <- Y; Z^T planes <- (M^T Y)^T; Y <- M Z (FP64 out)
+static int rf_iteration_planes(std::vector<Run> &rl, hipStream_t s) {
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        GTN_REQUIRE(f.p <= kRfTile, "gtn_svd_blocks: range-finder width above the plane tile");
+        R8N(f.Y, f.b.m * f.p, r.dws, r.n, "range-finder basis");
+        R8N((const double *)f.YTp, kRfTile * f.mp / 2, r.dws, r.n, "range-finder planes of Y");
+        LAUNCH(k_planes, dim3(kRfTile / 64, (unsigned)(f.mp / 64), r.n), dim3(256), 0, s, (const double *)f.Y, f.p,
+               f.b.m, f.p, f.mp, (int64_t)kRfTile, (__bf16 *)nullptr, f.YTp, r.dws, 4 * r.dws);
+    }
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        RfArgs a;
+        a.A = f.MTp, a.B = f.YTp, a.C = f.ZTp;
+        a.K = f.mp, a.plane_a = f.np * f.mp, a.plane_b = kRfTile * f.mp;
+        a.rows = f.b.n, a.rows_pad = f.np, a.cols = f.p, a.ldc = 0;
+        a.dA = a.dB = a.dC = 4 * r.dws, a.nb = r.n;
+        R8N((const double *)f.ZTp, kRfTile * f.np / 2, r.dws, r.n, "range-finder planes of Z");
+        LAUNCH(k_rf_nt<1>, rf_grid(f.np, r.n), dim3(256), 0, s, a);
+    }
+    return rf_y_from_zt(rl, s);
 }
 
 static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream_t s) {
@@ -1081,19 +1322,40 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
     // ---- large blocks: randomized subspace iteration
     if (!rl.empty()) {
         std::vector<gtn_gemm_desc> g;
+        const bool planes = planes_of(o);
+        if (planes) {
+            rc = rf_planes_of_blocks(rl, s);
+            if (rc) return rc;
+        }
         for (Run &r : rl) {
             Blk &f = *r.f;
             R8N(f.Z, f.b.n * f.p, r.dws, r.n, "gaussian");
             R8N(f.b.M, ext(f.b.m, f.b.n, f.b.lda), r.dM, r.n, "block");
             LAUNCH(k_fill_gaussian, grid2(f.b.n * f.p, r.n), dim3(256), 0, s, f.Z, f.b.n, f.p, f.p,
                    o.seed * 0x9E3779B97F4A7C15ull + (uint64_t)r.first, r.dws);
-            g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));  // Y = M Z
+            if (planes) {  // Z^T planes of the Gaussian block
+                R8N((const double *)f.ZTp, kRfTile * f.np / 2, r.dws, r.n, "range-finder planes of Z");
+                LAUNCH(k_planes, dim3(kRfTile / 64, (unsigned)(f.np / 64), r.n), dim3(256), 0, s, (const double *)f.Z,
+                       f.p, f.b.n, f.p, f.np, (int64_t)kRfTile, (__bf16 *)nullptr, f.ZTp, r.dws, 4 * r.dws);
+            } else {
+                g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));  // Y = M Z
+            }
         }
-        rc = run(g, o.precision, s);
+        rc = planes ? rf_y_from_zt(rl, s) : run(g, o.precision, s);
         if (rc) return rc;
         rc = orthonormalise(rl, o.iters > 0 ? 1 : 2, s);
         if (rc) return rc;
         for (int it = 0; it < o.iters; ++it) {
+            const int prec = it >= o.iters - o.f64_tail ? GTN_PREC_F64 : o.precision;
+            if (planes && prec != GTN_PREC_F64) {
+                rc = rf_iteration_planes(rl, s);
+                if (rc) return rc;
+                const int every = o.orth_every > 1 ? o.orth_every : 1;
+                rc = (it + 1 == o.iters) ? orthonormalise(rl, 2, s)
+                                         : ((it + 1) % every == 0 ? orthonormalise(rl, 1, s) : GTN_OK);
+                if (rc) return rc;
+                continue;
+            }
             g.clear();
             for (Run &r : rl) {
                 Blk &f = *r.f;
@@ -1102,7 +1364,7 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
                 else  // Z = M^T Y
                     g.push_back(gdr(f.b.M, 1, f.b.lda, f.Y, f.p, 1, f.Z, f.p, f.b.n, f.p, f.b.m, r.n, r.dM, r.dws, r.dws));
             }
-            rc = run(g, o.precision, s);
+            rc = run(g, prec, s);
             if (rc) return rc;
             if (!eig) {
                 g.clear();
@@ -1110,7 +1372,7 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
                     Blk &f = *r.f;
                     g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));
                 }
-                rc = run(g, o.precision, s);
+                rc = run(g, prec, s);
                 if (rc) return rc;
             } else {
                 for (Run &r : rl) {
@@ -1283,6 +1545,7 @@ extern "C" int gtn_svd_blocks(const gtn_svd_block *blocks, int n_blocks, const g
     GTN_REQUIRE(opts && (n_blocks == 0 || blocks), "gtn_svd_blocks: null argument");
     GTN_REQUIRE(opts->mode == 0 || opts->mode == 1, "gtn_svd_blocks: mode must be 0 (svd) or 1 (eig)");
     GTN_REQUIRE(opts->iters >= 0 && opts->iters <= 1000, "gtn_svd_blocks: iters out of range");
+    GTN_REQUIRE(opts->f64_tail >= 0, "gtn_svd_blocks: negative f64_tail");
     int64_t need = gtn_svd_workspace_bytes(blocks, n_blocks, opts);
     if (need > workspace_bytes || (need > 0 && !workspace)) {
         set_error("gtn_svd_blocks: workspace too small (%lld < %lld bytes)", (long long)workspace_bytes, (long long)need);

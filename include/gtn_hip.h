@@ -265,11 +265,12 @@ int gtn_odd_norm(const void *x_dev, int ndim, const int64_t *dims, const int32_t
                  const int32_t *pp, int dtype, double *out_dev, void *stream);
 /* Per-point normalisation of a batch of coarse-grained tensors with the driver's log-norm rule
  * (gauge2d.py:1746-1748 and example.py:191-193 for every point at once): for b < nbatch (<= 1024),
- * x_b = x[b n .. b n + n): nrm2[b] = ||x_b||^2 (FP64 accumulation), x_b /= ||x_b||,
- * lognorm[b] = 2 lognorm[b] + log ||x_b|| (when lognorm is not NULL), hist[b hist_len + k] =
- * log ||x_b|| with k = step[0] (when hist and step are given and k < hist_len), then step[0] += 1. */
-int gtn_normalize_batch(void *x_dev, int64_t nbatch, int64_t n, int dtype, double *nrm2_dev, double *lognorm_dev,
-                        double *hist_dev, int hist_len, double *step_dev, void *stream);
+ * x_b = x[b n .. b n + n): nrm2[b] = ||x_b||^2 (FP64 accumulation), x_b / ||x_b|| stored to dst_b
+ * (dst_dev NULL: in place into x_b), lognorm[b] = 2 lognorm[b] + log ||x_b|| (when lognorm is not
+ * NULL), hist[b hist_len + k] = log ||x_b|| with k = step[0] (when hist and step are given and
+ * k < hist_len), then step[0] += 1. */
+int gtn_normalize_batch(void *x_dev, void *dst_dev, int64_t nbatch, int64_t n, int dtype, double *nrm2_dev,
+                        double *lognorm_dev, double *hist_dev, int hist_len, double *step_dev, void *stream);
 /* The diagonal singular-value matrices of a batch of parity-blocked decompositions
  * (get_full_matrix of BlockSVD, __init__.py:4017-4028): dst (nbatch, 2d, 2d) with
  * dst[b][r][r] = src[b][r % 2][r / 2] (src (nbatch, 2, d): the even then the odd block's values) and
@@ -286,7 +287,9 @@ int gtn_diag_interleave(void *dst_dev, int64_t nbatch, int64_t d, const void *sr
  * Algorithm: randomized subspace iteration with block width p_b = k_b + oversample, `iters`
  * iterations, CholeskyQR2 re-orthonormalisation, then an FP64 Rayleigh-Ritz step (B = Q^T M,
  * eigen-decomposition of B B^T by cyclic Jacobi) -- the range finder may run in a reduced
- * `precision`; the Rayleigh-Ritz step is always FP64.
+ * `precision`, optionally with its last f64_tail iterations in FP64 (the iteration is a fixed-point
+ * map: the FP64 iterations damp the rounding error of the reduced ones outside the top-p
+ * subspace); the Rayleigh-Ritz step is always FP64.
  * Gauge: each column of U is signed so that its entry of largest magnitude is positive.
  * eig mode (mode=1): the block must be Hermitian (checked on the host path); eigenvalues are the
  * signed Rayleigh quotients lambda_k = u_k^T M u_k, ordered by |lambda|, and Vt = U^T.
@@ -304,7 +307,7 @@ typedef struct gtn_svd_opts {
     int32_t mode;       /* 0: SVD, 1: Hermitian eig */
     uint64_t seed;
     int32_t orth_every; /* re-orthonormalise the basis every this many iterations (<= 1: every one) */
-    int32_t pad_;
+    int32_t f64_tail;   /* the last f64_tail iterations run in FP64 whatever `precision` is (0: none) */
 } gtn_svd_opts;
 int64_t gtn_svd_workspace_bytes(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts);
 int gtn_svd_blocks(const gtn_svd_block *blocks, int n_blocks, const gtn_svd_opts *opts, void *workspace_dev,

@@ -373,8 +373,9 @@ class FakeLib:
         self._arr(out, 1, np.float64, "odd out")[0] = float(np.sum(np.abs(v[(par % 2) == 1]) ** 2))
         return 0
 
-    def gtn_normalize_batch(self, x, nb, n, dtype, nrm2, lognorm, hist, hist_len, step, stream):
+    def gtn_normalize_batch(self, x, dst, nb, n, dtype, nrm2, lognorm, hist, hist_len, step, stream):
         v = self._arr(x, nb * n, NP[dtype], "normalize x").reshape(nb, n)
+        out = self._arr(dst, nb * n, NP[dtype], "normalize dst").reshape(nb, n) if dst else v
         q = (v.astype(np.float64) ** 2).sum(axis=1)
         self._arr(nrm2, nb, np.float64, "normalize nrm2")[:] = q
         ln = 0.5 * np.log(q)
@@ -386,7 +387,7 @@ class FakeLib:
             if 0 <= k < hist_len:
                 h = self._arr(hist, nb * hist_len, np.float64, "normalize hist").reshape(nb, hist_len)
                 h[:, k] = ln
-        v /= np.sqrt(q)[:, None].astype(v.dtype)
+        out[...] = v / np.sqrt(q)[:, None].astype(v.dtype)
         if step:
             self._arr(step, 1, np.float64, "normalize step")[0] += 1
         return 0
