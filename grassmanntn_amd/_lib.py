@@ -112,6 +112,7 @@ def _bind(lib):
         "gtn_event_destroy": (C.c_int, [vp]),
         "gtn_index_tables": (C.c_int, [i64, vp, vp, vp, vp]),
         "gtn_permute": (C.c_int, [C.POINTER(PermDesc), vp, vp, vp]),
+        "gtn_permute2": (C.c_int, [C.POINTER(PermDesc), C.POINTER(PermDesc), vp, vp, vp]),
         "gtn_sign_tensor": (C.c_int, [i32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), vp, vp]),
         "gtn_gemm_grouped": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),
         "gtn_gemm_grouped_noalloc": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),

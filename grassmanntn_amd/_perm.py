@@ -114,6 +114,25 @@ class Perm:
         return dst
 
 
+def run_composed(outer, inner, src, dst=None, alpha=1.0):
+    """outer(inner(src)) in one pass (gtn_permute2): `outer` is built against the contiguous tensor
+    that `inner` would write."""
+    lib = L.lib()
+    if dst is None:
+        dst = DeviceArray.empty(outer.out_shape(), src.dtype)
+    if dst.numel() == 0:
+        return dst
+    if not (src.is_contiguous() and dst.is_contiguous()):
+        raise ValueError("gather operands must be contiguous")
+    mid = 1
+    for d in inner.out_shape():
+        mid *= d
+    do = outer.desc(L.dtype_code(src), mid, alpha)
+    dn = inner.desc(L.dtype_code(src), src.numel())
+    L.check(lib.gtn_permute2(C.byref(do), C.byref(dn), L.ptr(src), L.ptr(dst), L.stream_ptr()), "gtn_permute2")
+    return dst
+
+
 def strides(shape):
     st, acc = [0] * len(shape), 1
     for i in range(len(shape) - 1, -1, -1):

@@ -85,16 +85,23 @@ def svd_batch(calls, cutoff):
         left, right = parse_partition(string, "svd")
         z = fresh_char(left + right)
         P = T.shape[0]
-        M = join_legs(T, "(%s)(%s)(%s)" % (z, left, right), "matrix", intermediate_stat=(0, -1, 1))
-        if M.encoder != "parity-preserving":
-            M = M.force_encoder("parity-preserving")
-        _, m, n = M.shape
+        spec = "(%s)(%s)(%s)" % (z, left, right)
+        try:
+            # the join gather writes the parity blocks of every point's matrix directly
+            G, src, (_, m, n), mstats = core._join_gather(T, spec, "matrix", (0, -1, 1), blocks=True)
+            M = None
+        except ValueError:  # a group mixing bosonic and fermionic legs: join, then cut the blocks
+            G, src = None, None
+            M = join_legs(T, spec, "matrix", intermediate_stat=(0, -1, 1))
+            _, m, n = M.shape
+            mstats = M.statistics
         mh, nh = m // 2, n // 2
         k = min(mh, nh, int(cutoff / 2))
         if k < 1:
             raise ValueError("svd_batch: cutoff %s keeps no values" % cutoff)
         d = _bond(k) // 2
-        jobs.append(dict(T=T, left=left, right=right, z=z, P=P, M=M, m=m, n=n, mh=mh, nh=nh, k=k, d=d))
+        jobs.append(dict(T=T, left=left, right=right, z=z, P=P, M=M, G=G, src=src, mstats=tuple(mstats), m=m, n=n,
+                         mh=mh, nh=nh, k=k, d=d))
     # calls of equal block shape share one buffer each for blocks, U, S and V, so that the device
     # decomposition sees all their blocks at constant strides (one launch per stage)
     shapes = {}
@@ -108,7 +115,10 @@ def svd_batch(calls, cutoff):
         for j in js:
             P = j["P"]
             j["blocks"] = blocks.flat(at * 2 * mh * nh, P * 2 * mh * nh, (P, 2, mh, nh))
-            _blocks_batch(j["M"].data, P, j["m"], j["n"], out=j["blocks"])
+            if j["G"] is not None:
+                j["G"].run(j["src"], dst=j["blocks"])
+            else:
+                _blocks_batch(j["M"].data, P, j["m"], j["n"], out=j["blocks"])
             j["SU"] = SU.flat(at * 2 * mh * d, P * 2 * mh * d, (P, 2, mh, d))
             j["SV"] = SV.flat(at * 2 * d * nh, P * 2 * d * nh, (P, 2, d, nh))
             j["sv"] = sv.flat(at * 2 * d, P * 2 * d, (P, 2, d))
@@ -146,7 +156,7 @@ def svd_batch(calls, cutoff):
         S = DeviceArray.empty((P, 2 * d, 2 * d))
         L.check(lib.gtn_diag_interleave(L.ptr(S), P, d, L.ptr(j["sv"]), L.GTN_F64, L.stream_ptr()),
                 "gtn_diag_interleave")
-        s0, s1 = j["M"].statistics[1], j["M"].statistics[2]
+        s0, s1 = j["mstats"][1], j["mstats"][2]
         U, S, V = _mk(U, (0, s0, 1)), _mk(S, (0, -1, 1)), _mk(V, (0, -1, s1))
         nl = len(left)
         xstats, xshape = T.statistics[1:], T.shape[1:]
@@ -181,8 +191,11 @@ def trg_batch_step(Tb, dcut):
     r = core.sqrt(S2)
     U2 = einsum("Babx,Bxc->Babc", U2, r)
     V2 = einsum("Bax,Bxbc->Babc", r, V2)
-    VV = einsum("Bkwz,Blxw->Blxzk", V1, V2)
-    UU = einsum("Byxi,Bzyj->Bjzxi", U1, U2)
+    # VV and UU stay in their GEMMs' parity-block layout: their output gathers run inside the
+    # operand gathers of the T' contraction (one pass over each instead of two)
+    with core.deferred_outputs():
+        VV = einsum("Bkwz,Blxw->Blxzk", V1, V2)
+        UU = einsum("Byxi,Bzyj->Bjzxi", U1, U2)
     return einsum("Blxzk,Bjzxi->Bijkl", VV, UU), sv
 
 

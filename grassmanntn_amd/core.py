@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib as L
 from ._device import DeviceArray, copy2d
-from ._perm import Perm, strides
+from ._perm import Perm, strides, run_composed
 from ._plan import (BOSE, FERMI, HYBRID, SEPARATORS, Contraction, GrassmannError, contraction, denumerate, fail,
                     fresh_char, group_info, parse_partition, sign_table)
 
@@ -672,6 +672,54 @@ class _PermutedView(dense):
     ndim = property(lambda self: len(self._order))
 
 
+class _PendingOutput(dense):
+    """A contraction result whose output gather is deferred (einsum inside `deferred_outputs()`):
+    `_src` holds the GEMM's parity-block result and `_perm` the gather that makes the dense tensor.
+    The operand gather of a following contraction composes with it (one pass over the data,
+    gtn_permute2); any other use materialises the tensor on first access of `.data`."""
+
+    def __init__(self, perm, src, stats, encoder, fmt, even):
+        self._perm, self._src = perm, src
+        self._shape = tuple(perm.out_shape())
+        self._data = None
+        self.statistics = tuple(stats)
+        self.encoder, self.format = encoder, fmt
+        self._even = even
+
+    @property
+    def data(self):
+        if self._data is None:
+            self._data = self._perm.run(self._src)
+        return self._data
+
+    @data.setter
+    def data(self, value):
+        self._data = value
+
+    shape = property(lambda self: self._shape if self._data is None else tuple(self._data.shape))
+    ndim = property(lambda self: len(self.shape))
+
+
+_defer = [False]
+
+
+class deferred_outputs:
+    """Context manager: two-operand contractions inside return _PendingOutput results, whose output
+    gather merges into the next contraction's operand gather."""
+
+    def __enter__(self):
+        self._old = _defer[0]
+        _defer[0] = True
+        return self
+
+    def __exit__(self, *a):
+        _defer[0] = self._old
+
+
+def _source_dtype(T):
+    return T._src.dtype if isinstance(T, _PendingOutput) and T._data is None else T.data.dtype
+
+
 def _gather_permuted(src, order, codes, fbits):
     """out[x] = code(parities of x) * src[x permuted]: out axis a is source axis order[a]; bit k of the
     table index is the parity of out axis fbits[k]."""
@@ -844,6 +892,8 @@ def _operand_gather(T, s, plan, layout, fchars, codes, sig_chars):
             else:
                 P.add_out(full, [(virt_of[("F", item[1])], 1)])
     P.set_table(codes)
+    if isinstance(T, _PendingOutput) and T._data is None:
+        return run_composed(P, T._perm, T._src)
     return P.run(T.data)
 
 
@@ -853,9 +903,10 @@ def _einsum2(plan, A, B, first, ignore):
     dims = plan.dims
     setA, setB, setO = set(sA), set(sB), set(out)
     uA, uB = list(dict.fromkeys(sA)), list(dict.fromkeys(sB))
-    if A.data.is_complex() or B.data.is_complex():
+    dA, dB = _source_dtype(A), _source_dtype(B)
+    if np.dtype(dA).kind == "c" or np.dtype(dB).kind == "c":
         raise NotImplementedError("complex contractions are not provided by this build")
-    if A.data.dtype != B.data.dtype:
+    if dA != dB:
         raise NotImplementedError("operands of different dtypes")
     isF = lambda c: _fermionic(plan, c)
     batch = [c for c in uA if c in setB and c in setO]
@@ -987,6 +1038,8 @@ def _einsum2(plan, A, B, first, ignore):
         P.add_comp(v, dims[c], [(i, m)], bit=fo.index(c) if c in fo else -1)
         P.add_out(dims[c], [(v, 1)])
     P.set_table(codesO)
+    if _defer[0]:
+        return _PendingOutput(P, Cg, plan.out_stats, first.encoder, first.format, True if blocked else None)
     res = dense.__new__(dense)
     res.data = P.run(Cg)
     res.statistics = plan.out_stats
@@ -1062,6 +1115,19 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
     Bosons move left inside a group, +1 legs in a -1 group pick up (-1)^p, the fermionic legs fuse in
     C order (first leg = high bits), matrix format applies sigma to each fused -1 leg, then the
     encoder switches on the fused leg."""
+    P, src_data, fin_shape, fin_stats = _join_gather(T, string_inp, make_format, intermediate_stat)
+    out = T._like(P.run(src_data), statistics=fin_stats, encoder="parity-preserving", format=make_format,
+                  even=T._even)
+    out.data = out.data.reshape(fin_shape)
+    return out
+
+
+def _join_gather(T, string_inp, make_format="standard", intermediate_stat=None, blocks=False):
+    """The gather of join_legs: (Perm, source data, fused shape, fused statistics).  With blocks=True
+    (a bosonic first group, then two purely fermionic groups: the batched matrices M[b] of a
+    decomposition) the gather writes the two parity blocks M[b][p::2, p::2] instead, as a
+    (B, 2, m/2, n/2) array (BlockSVD's ME / MO, __init__.py:3990-3996): the parity p is an output axis
+    of its own feeding both fused legs."""
     string_inp = denumerate(string_inp.replace(" ", ""))
     inter = tuple(intermediate_stat) if not np.isscalar(intermediate_stat) else (intermediate_stat,)
     groups, srt = group_info(string_inp, T.statistics, T.shape)
@@ -1086,6 +1152,12 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
     nb = len(bit_axes)
     bitpos = {p: k for k, p in enumerate(bit_axes)}
     fin_shape, fin_stats = [], []
+    if blocks:
+        ok = (len(srt) == 3 and info[0][3] == 0 and all(info[g][2] == 0 and info[g][3] and info[g][1] % 2 == 0
+                                                          for g in (1, 2)))
+        if not ok:
+            raise ValueError("_join_gather: blocks=True needs a bosonic group and two fermionic groups")
+        fvirt = []
     for g, (chars, stg, dm, pos) in enumerate(srt):
         bd, fd, bn, fn, final = info[g]
         # the fused leg: bosons (high) then fermions; the fermionic part is switched to the
@@ -1106,7 +1178,15 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
             vf = P.add_virt(fd, enc=inter[g] in FERMI, sigma=(make_format == "matrix" and inter[g] == -1))
             for t, d, p in zip(stg, dm, pos):
                 P.add_comp(vf, d, [(ins[p], 1)], bit=bitpos.get(p, -1))
-            P.add_out(fd, [(vf, 1)])
+            if blocks:
+                fvirt.append((vf, fd))
+                if len(fvirt) == 2:  # (parity, row / 2, column / 2): index = 2 half + parity
+                    (vr, mr), (vc, nc) = fvirt
+                    P.add_out(2, [(vr, 1), (vc, 1)])
+                    P.add_out(mr // 2, [(vr, 2)])
+                    P.add_out(nc // 2, [(vc, 2)])
+            else:
+                P.add_out(fd, [(vf, 1)])
         else:
             # a purely bosonic group is neither encoded nor sigma-scaled (its intermediate leg has
             # statistic 0) but ends up labelled with the requested intermediate statistic
@@ -1125,10 +1205,7 @@ def join_legs(T, string_inp, make_format="standard", intermediate_stat=None, sav
                 code = 2 if t == 2 else code ^ t
             table.append(code)
         P.set_table(table)
-    out = T._like(P.run(src_data), statistics=fin_stats, encoder="parity-preserving", format=make_format,
-                  even=T._even)
-    out.data = out.data.reshape(fin_shape)
-    return out
+    return P, src_data, fin_shape, fin_stats
 
 
 @_sparse_in

@@ -241,20 +241,22 @@ __device__ __forceinline__ int64_t map_elem(const PermArgs &a, const TabArgs &t,
         return perm_map(a, lin, code);
 }
 
-// kPermUnroll output elements per lane and pass (a grid stride apart, so each pass stays coalesced):
-// their index maps and table reads are issued together, then their source reads, so a lane keeps
-// kPermUnroll dependent load chains in flight instead of one.
+// A workgroup takes kPermUnroll * 256 consecutive output elements per pass, a lane every 256th of
+// them (each load instruction stays coalesced, and the workgroups sweep the output in order, which
+// keeps the source lines they share in L2); a lane issues the index maps and table reads of its
+// kPermUnroll elements together, then their source reads, so kPermUnroll dependent load chains are in
+// flight per lane instead of one.
 constexpr int kPermUnroll = 4;
 template <typename T, bool TAB>
 __global__ void __launch_bounds__(256) k_permute(PermArgs a, TabArgs tb, const T *__restrict__ src, T *__restrict__ dst) {
     const double al = a.alpha_re;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < a.total; i0 += kPermUnroll * stride) {
+    const int64_t chunk = (int64_t)kPermUnroll * blockDim.x;
+    for (int64_t i0 = blockIdx.x * chunk + threadIdx.x; i0 < a.total; i0 += (int64_t)gridDim.x * chunk) {
         int64_t off[kPermUnroll];
         uint32_t code[kPermUnroll];
 #pragma unroll
         for (int u = 0; u < kPermUnroll; ++u) {
-            const int64_t i = i0 + u * stride;
+            const int64_t i = i0 + u * (int64_t)blockDim.x;
             code[u] = 2u;
             off[u] = 0;
             if (i < a.total) off[u] = map_elem<TAB>(a, tb, (uint64_t)i, code[u]);
@@ -264,8 +266,44 @@ __global__ void __launch_bounds__(256) k_permute(PermArgs a, TabArgs tb, const T
         for (int u = 0; u < kPermUnroll; ++u) v[u] = (code[u] == 2u) ? 0.0 : (double)src[off[u]];
 #pragma unroll
         for (int u = 0; u < kPermUnroll; ++u) {
-            const int64_t i = i0 + u * stride;
+            const int64_t i = i0 + u * (int64_t)blockDim.x;
             if (i < a.total) dst[i] = (T)(code[u] == 2u ? 0.0 : v[u] * (code[u] ? -al : al));
+        }
+    }
+}
+
+// Two gathers in one pass: dst[i] = sign_o(i) sign_n(j) src[off_n(j)] with j = off_o(i), the outer
+// gather's source offset, read as the inner gather's output index (the inner gather's output is the
+// contiguous tensor the outer one would have read).  Same lane arrangement as k_permute.
+template <typename T, bool TABO, bool TABN>
+__global__ void __launch_bounds__(256) k_permute2(PermArgs ao, TabArgs to, PermArgs an, TabArgs tn,
+                                                  const T *__restrict__ src, T *__restrict__ dst) {
+    const double al = ao.alpha_re * an.alpha_re;
+    const int64_t chunk = (int64_t)kPermUnroll * blockDim.x;
+    for (int64_t i0 = blockIdx.x * chunk + threadIdx.x; i0 < ao.total; i0 += (int64_t)gridDim.x * chunk) {
+        int64_t off[kPermUnroll];
+        uint32_t code[kPermUnroll];
+#pragma unroll
+        for (int u = 0; u < kPermUnroll; ++u) {
+            const int64_t i = i0 + u * (int64_t)blockDim.x;
+            code[u] = 2u;
+            off[u] = 0;
+            if (i < ao.total) {
+                uint32_t co, cn;
+                const int64_t j = map_elem<TABO>(ao, to, (uint64_t)i, co);
+                if (co != 2u) {
+                    off[u] = map_elem<TABN>(an, tn, (uint64_t)j, cn);
+                    code[u] = cn == 2u ? 2u : (co ^ cn);
+                }
+            }
+        }
+        double v[kPermUnroll];
+#pragma unroll
+        for (int u = 0; u < kPermUnroll; ++u) v[u] = (code[u] == 2u) ? 0.0 : (double)src[off[u]];
+#pragma unroll
+        for (int u = 0; u < kPermUnroll; ++u) {
+            const int64_t i = i0 + u * (int64_t)blockDim.x;
+            if (i < ao.total) dst[i] = (T)(code[u] == 2u ? 0.0 : v[u] * (code[u] ? -al : al));
         }
     }
 }
@@ -1002,6 +1040,52 @@ int gtn_permute(const gtn_perm_desc *d, const void *src, void *dst, void *stream
         case GTN_C128: hipLaunchKernelGGL(k_permute_c128, g, b, 0, s, a, (const double2 *)src, (double2 *)dst); break;
         default: set_error("gtn_permute: unsupported dtype %d", d->dtype); return GTN_ENOTSUP;
     }
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
+int gtn_permute2(const gtn_perm_desc *outer, const gtn_perm_desc *inner, const void *src, void *dst, void *stream) {
+    GTN_REQUIRE(outer && inner && src && dst, "gtn_permute2: null argument");
+    GTN_REQUIRE(outer->dtype == inner->dtype && (outer->dtype == GTN_F64 || outer->dtype == GTN_F32),
+                "gtn_permute2: both gathers must be F64 or both F32");
+    PermArgs ao, an;
+    int rc = build_perm_args(outer, ao);
+    if (rc) return rc;
+    rc = build_perm_args(inner, an);
+    if (rc) return rc;
+    GTN_REQUIRE(outer->src_elems == an.total, "gtn_permute2: the outer gather reads %lld elements, the inner one makes %lld",
+                (long long)outer->src_elems, (long long)an.total);
+    if (ao.total == 0) return GTN_OK;
+    if (dry_run()) {
+        const int es = outer->dtype == GTN_F32 ? 4 : 8;
+        GTN_RANGE(src, inner->src_elems * es, "permute2 src");
+        GTN_RANGE(dst, ao.total * es, "permute2 dst");
+        return GTN_OK;
+    }
+    hipStream_t s = as_stream(stream);
+    TabArgs to, tn;
+    memset(&to, 0, sizeof(to));
+    memset(&tn, 0, sizeof(tn));
+    TabBuild bo, bn;
+    const bool tabs = getenv("GTN_PERM_NO_TABLES") == nullptr;
+    const bool tabo = tabs && plan_groups(outer, ao.total, bo, to);
+    const bool tabn = tabs && plan_groups(inner, an.total, bn, tn);
+    if (tabo && (rc = tab_tables(ao, bo, s, to))) return rc;
+    if (tabn && (rc = tab_tables(an, bn, s, tn))) return rc;
+    dim3 g(grid_for((ao.total + kPermUnroll - 1) / kPermUnroll, 256, 1 << 20)), b(256);
+#define GTN_P2(T, A, B) hipLaunchKernelGGL((k_permute2<T, A, B>), g, b, 0, s, ao, to, an, tn, (const T *)src, (T *)dst)
+    if (outer->dtype == GTN_F64) {
+        if (tabo && tabn) GTN_P2(double, true, true);
+        else if (tabo) GTN_P2(double, true, false);
+        else if (tabn) GTN_P2(double, false, true);
+        else GTN_P2(double, false, false);
+    } else {
+        if (tabo && tabn) GTN_P2(float, true, true);
+        else if (tabo) GTN_P2(float, true, false);
+        else if (tabn) GTN_P2(float, false, true);
+        else GTN_P2(float, false, false);
+    }
+#undef GTN_P2
     GTN_CHECK_LAUNCH();
     return GTN_OK;
 }

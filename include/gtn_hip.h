@@ -161,6 +161,13 @@ typedef struct gtn_perm_desc {
     int64_t src_elems; /* number of elements addressable from src (checked on the host) */
 } gtn_perm_desc;
 int gtn_permute(const gtn_perm_desc *desc, const void *src_dev, void *dst_dev, void *stream);
+/* Two gathers composed in one pass (no counterpart in the reference, whose contractions materialise
+ * every intermediate): dst = outer(inner(src)), i.e. dst[i] = sign_outer(i) * sign_inner(j) *
+ * src[off_inner(j)] with j = off_outer(i), the outer gather reading the contiguous tensor the inner
+ * one would write (outer->src_elems must equal the inner gather's output size); both F64 or both
+ * F32; alpha multiplies. */
+int gtn_permute2(const gtn_perm_desc *outer, const gtn_perm_desc *inner, const void *src_dev, void *dst_dev,
+                 void *stream);
 
 /* ------------------------------------------------------------------------------------------------
  * (A3) sign tensor, as the reference materialises S1 / S3 (__init__.py:1944-1999, :2088-2126):

@@ -186,6 +186,25 @@ class FakeLib:
     def gtn_permute(self, dp, src, dst, stream):
         d = dp._obj
         self.calls.append(("permute", d.n_out))
+        return self._gather(d, lambda n: self._arr(src, n, NP[d.dtype], "permute src"), dst)
+
+    def gtn_permute2(self, dpo, dpn, src, dst, stream):
+        do, dn = dpo._obj, dpn._obj
+        self.calls.append(("permute2", do.n_out, dn.n_out))
+        if do.dtype != dn.dtype:
+            return self._fail("permute2: dtypes differ")
+        mid_n = 1
+        for a in range(dn.n_out):
+            mid_n *= dn.out_dim[a]
+        if do.src_elems != mid_n:
+            return self._fail("permute2: the outer gather reads %d elements, the inner one makes %d" % (do.src_elems, mid_n))
+        mid = np.zeros(mid_n, dtype=NP[dn.dtype])
+        self._gather(dn, lambda n: self._arr(src, n, NP[dn.dtype], "permute2 src"), None, out=mid)
+        return self._gather(do, lambda n: mid[:n], dst)
+
+    def _gather(self, d, src_of, dst, out=None):
+        """The signed gather of descriptor d: src_of(n) gives the first n source elements; the result
+        goes to device address dst (or the host array `out`)."""
         n_out, n_virt, n_in = d.n_out, d.n_virt, d.n_in
         comps = []
         for v in range(n_virt):
@@ -196,7 +215,7 @@ class FakeLib:
         for a in range(n_out):
             total *= d.out_dim[a]
         dt = NP[d.dtype]
-        dst_a = self._arr(dst, total, dt, "permute dst")
+        dst_a = out if out is not None else self._arr(dst, total, dt, "permute dst")
         if total == 0:
             return 0
         lin = np.arange(total, dtype=np.uint64)
@@ -252,7 +271,7 @@ class FakeLib:
         sign = np.where((code ^ neg) & np.uint64(1), -1.0, 1.0)
         if off.min() < 0 or off.max() >= d.src_elems:
             raise AssertionError("permute reads outside src_elems")
-        src_a = self._arr(src, int(off.max()) + 1, dt, "permute src")
+        src_a = src_of(int(off.max()) + 1)
         alpha = complex(d.alpha_re, d.alpha_im) if d.dtype == 2 else d.alpha_re
         vals = src_a[off] * sign * alpha
         vals[zero] = 0
