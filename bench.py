@@ -137,16 +137,20 @@ def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary (FETCH_SIZE and
-    WRITE_SIZE passes scaled by a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "tprime_gemm_d64_chi64_hbm_bytes.txt")
-    if not os.path.exists(path):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summaries
+    (profiles/*_hbm_bytes.txt: FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known
+    size), or None."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
         return None
-    for line in open(path):
-        if line.startswith("machine-readable: "):
-            d = json.loads(line[len("machine-readable: "):])
-            if d.get("kernel") == kernel:
-                return d["traffic_bytes"]
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith("_hbm_bytes.txt"):
+            continue
+        for line in open(os.path.join(pdir, name)):
+            if line.startswith("machine-readable: "):
+                d = json.loads(line[len("machine-readable: "):])
+                if d.get("kernel") == kernel:
+                    return d["traffic_bytes"]
     return None
 
 
@@ -347,6 +351,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the copy kernel over this many MiB (HBM counter calibration)")
+    ap.add_argument("--rf-gemm-only", action="store_true",
+                    help="only time the range finder's FP64 GEMM shape (the roofline kernel; counter runs)")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="f32", help="contraction arithmetic of the mixed path")
@@ -379,6 +385,9 @@ def main():
     import grassmanntn_amd as gtn
     if args.copy_calibration_mib:
         copy_calibration(args.copy_calibration_mib)
+        return
+    if args.rf_gemm_only:
+        print(json.dumps({"roofline": rf_gemm_roofline(args.batch)}))
         return
     if args.svd_iters:
         gtn.settings.svd_iters = args.svd_iters
