@@ -7,10 +7,10 @@ Grassmann-even site tensors (entries N(0,1) times exp(-0.5 * sum of leg popcount
 zeroed, max|T| = 1, parity-preserving legs, statistics (1,1,-1,-1); seeds 1..P) coarse-grained
 together for --steps (20) steps, every step one replay of a recorded HIP graph; arithmetic
 (gtn.mixed_schedule): contractions FP64 in the first five steps, then FP32 MFMA with FP32
-accumulation; in every step the parity-block SVD's range finder runs in split BF16 (three BF16
-products, FP32 accumulation) except its last three iterations, which are FP64, while the
-Rayleigh-Ritz step, the singular vectors, the normalisation and the log-norm sum are FP64.  Steps/s
-counts (point, step) pairs.  Beside it: the plain FP64 GPU path (one tensor, the eager gtn.trg loop)
+accumulation; the parity-block SVD (range finder, Rayleigh-Ritz, vectors), the normalisation and the
+log-norm sum FP64 at every step.  Steps/s counts (point, step) pairs.  Beside it: the same batch with
+other range-finder arithmetic (split BF16 with FP64 closing iterations; FP64 on int8 slices), each
+with its speed-up and ln Z error, the plain FP64 GPU path (one tensor, the eager gtn.trg loop)
 and the FP64 batched path (same batch, all FP64), ln Z per site of every point with an oracle golden
 against that golden, the roofline of the step's dominant kernel (the range finder's FP64 GEMM, timed
 with HIP events on the same shape), the coupling sweep (512 points, leg-dim-32 generators, chi from
@@ -357,7 +357,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="f32", help="contraction arithmetic of the mixed path")
     ap.add_argument("--svd-precision", default=None, help="range-finder arithmetic of the mixed path "
-                    "(default: the library's mixed schedule, split BF16 with three FP64 iterations at the end)")
+                    "(default: the library's mixed schedule, FP64)")
     ap.add_argument("--fp64-steps", type=int, default=5,
                     help="leading TRG steps of the mixed path run in FP64 (their normalisation errors "
                          "weigh 2^-n in ln Z per site)")
@@ -366,6 +366,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 512-point coupling sweep")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned surrogate")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rf-variants", action="store_true", help="skip the mixed path with other range-finder "
+                    "arithmetic")
     ap.add_argument("--no-einsum", action="store_true", help="skip the standalone 'ijkl,klmn->ijmn' "
                     "contraction of two 64^4 tensors")
     ap.add_argument("--verbose", action="store_true", help="per-step progress and norms on stderr")
@@ -416,8 +418,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 contractions in steps 1-%d, then %s; range finder %s; f64 Rayleigh-Ritz, normalisation "
-                 "and log-norm sum" % (args.fp64_steps, prec, mixed_sched(0)[1]),
+        "dtype": "f64 contractions in steps 1-%d, then %s; SVD (range finder %s), normalisation and log-norm "
+                 "sum f64" % (args.fp64_steps, prec, mixed_sched(0)[1]),
         "data": "synthetic",
         "config": {"workload": "TRG at leg dim 64, chi 64: %d seeded site tensors (seeds %d..%d) coarse-grained "
                                "together for %d steps, one HIP-graph replay per step" % (P, SEED, SEED + P - 1, K),
@@ -434,6 +436,25 @@ def main():
         out["dlnz_mixed_vs_oracle"] = max(errs.values())
     else:
         out["oracle_check"] = "skipped: the goldens cover 20 steps, %d were timed" % K
+    if not args.no_rf_variants:
+        # the same batch and contraction schedule with other range-finder arithmetic
+        out["range_finder_variants"] = {}
+        for spec, what in (("bf16x3:3", "split BF16 (three BF16 products, FP32 accumulation) except the last "
+                                        "three iterations, FP64; Rayleigh-Ritz FP64"),
+                           ("f64_int8", "FP64 products on the int8 matrix cores (eight 7-bit slices per operand "
+                                        "row, exact integer products)")):
+            if spec == svd_prec:
+                continue
+            progress("mixed batched path, range finder %s" % spec)
+            sched = lambda i, spec=spec: gtn.mixed_schedule(i, prec, svd=spec, fp64_steps=args.fp64_steps)
+            var = batched_path(gtn, P, K, args.warmup, sched, T0s)
+            v = {"range_finder": what, "steps_per_s": P * K / (var["gpu_ms"] / 1000.0),
+                 "ms_per_point_step": var["gpu_ms"] / (P * K)}
+            if golden:
+                e = {str(s): abs(var["lnz"][s - SEED] - g) for s, g in golden.items() if s - SEED < P}
+                v["dlnz_vs_oracle_by_seed"] = e
+                v["dlnz_vs_oracle"] = max(e.values())
+            out["range_finder_variants"][spec] = v
     if not args.no_fp64:
         progress("FP64 batched path")
         f64b = batched_path(gtn, P, K, args.warmup, lambda i: ("f64", "f64"), T0s)
@@ -446,6 +467,8 @@ def main():
         plain = run_path(gtn, "f64", "f64", K, args.warmup, T0s[0])
         out["fp64_steps_per_s"] = K / (plain["gpu_ms"] / 1000.0)
         out["speedup_mixed_vs_fp64"] = value / out["fp64_steps_per_s"]
+        for v in out.get("range_finder_variants", {}).values():
+            v["speedup_vs_fp64"] = v["steps_per_s"] / out["fp64_steps_per_s"]
         out["fp64_path"] = "plain FP64 GPU path: one tensor (seed %d), the eager gtn.trg loop, %d steps" % (SEED, K)
         out["recomputed_steps_fp64"] = plain["recomputed_steps"]
         if SEED in golden:

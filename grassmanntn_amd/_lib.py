@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libgtn_hip.so")
 
 GTN_F64, GTN_F32, GTN_C128, GTN_I8 = 0, 1, 2, 3
 GTN_ACT_NONE, GTN_ACT_GELU = 0, 1
-PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16": 4, "bf16_split_a": 5}
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16": 4, "bf16_split_a": 5, "f64_int8": 6}
 MAX_AX, MAX_COMP, MAX_BITS = 10, 16, 10
 
 
@@ -118,6 +118,7 @@ def _bind(lib):
         "gtn_gemm_grouped_noalloc": (C.c_int, [C.POINTER(GemmDesc), i32, i32, i32, vp]),
         "gtn_gemm_noalloc_max_scratch": (C.c_int64, []),
         "gtn_mfma_selftest": (C.c_int, [vp, vp]),
+        "gtn_mfma_selftest_i8": (C.c_int, [vp, vp]),
         "gtn_autotune_set": (C.c_int, [i32]),
         "gtn_autotune_clear": (C.c_int, []),
         "gtn_autotune_report": (C.c_int, [C.POINTER(TuneEntry), i32]),

@@ -78,8 +78,8 @@ class precision:
     def __init__(self, gemm="f64", svd=None):
         if svd is not None:
             svd_spec(svd)
-        if gemm not in L.PREC:
-            raise ValueError("precision must be one of %s" % list(L.PREC))
+        if gemm not in L.PREC or gemm == "f64_int8":
+            raise ValueError("precision must be one of %s" % [k for k in L.PREC if k != "f64_int8"])
         self.gemm, self.svd = gemm, svd
 
     def __enter__(self):
@@ -94,7 +94,7 @@ class precision:
 
 
 MIXED_FP64_STEPS = 5
-MIXED_SVD = "bf16x3:3"
+MIXED_SVD = "f64"
 
 
 def mixed_schedule(step, contraction="f32", svd=MIXED_SVD, fp64_steps=MIXED_FP64_STEPS):
@@ -102,12 +102,11 @@ def mixed_schedule(step, contraction="f32", svd=MIXED_SVD, fp64_steps=MIXED_FP64
     schedule.  Contractions: FP64 in the first `fp64_steps` steps, then `contraction`.  The FP64
     leading steps are there because ln Z per site weighs the normalisation of step n by 2^-n
     (example.py:191-193): FP32 rounding (relative ~1e-7 in a step's norm) costs ~1e-8 when it starts
-    at step 4 and ~4e-9 from step 6 on (measured at leg dims 16 and 64).  Decompositions, every step:
-    the range finder in split BF16 except its last three iterations, which run in FP64 and damp the
-    BF16 rounding left in the subspace (see svd_spec); the Rayleigh-Ritz step, the singular vectors,
-    normalisation and the log-norm sum are FP64.  At leg dim 64, chi 64 with FP64 contractions this
-    range finder moves ln Z per site by at most 2.6e-10 over four seeds (two FP64 iterations: 1.8e-9,
-    six: 2.4e-11)."""
+    at step 4 and ~4e-9 from step 6 on (measured at leg dims 16 and 64).  Decompositions: `svd` (see
+    svd_spec), by default FP64 throughout; the normalisation and the log-norm sum are FP64.  Other
+    range-finder arithmetic measured at leg dim 64, chi 64 with FP64 contractions, largest
+    |d ln Z/site| over four seeds: split BF16 with its last three iterations FP64 ('bf16x3:3')
+    2.6e-10 (two: 1.8e-9, six: 2.4e-11); FP64 on int8 slices ('f64_int8') as the native FP64."""
     return ("f64" if step < fp64_steps else contraction, svd)
 
 

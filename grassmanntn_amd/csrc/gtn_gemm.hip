@@ -876,6 +876,32 @@ extern "C" int gtn_gemm_grouped_noalloc(const gtn_gemm_desc *groups, int n_group
                              "gtn_gemm_grouped_noalloc", tile, target);
 }
 
+// int8 16x16x64 lane-layout self-test (the operands are deterministic small integers; one wavefront)
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(64) k_mfma_selftest_i8(int32_t *out) {
+    const int l = threadIdx.x;
+    if (l >= 64) return;
+    int8_t av[16], bv[16];
+    for (int t = 0; t < 16; ++t) {
+        const int k = 16 * (l / 16) + t;
+        av[t] = (int8_t)(((l % 16) + 3 * k) % 7 - 3);
+        bv[t] = (int8_t)((2 * (l % 16) + k) % 5 - 2);
+    }
+    i32x4 a, b;
+    memcpy(&a, av, 16);
+    memcpy(&b, bv, 16);
+    i32x4 acc = {0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) out[4 * l + v] = acc[v];
+}
+
+extern "C" int gtn_mfma_selftest_i8(int32_t *out_dev, void *stream) {
+    GTN_REQUIRE(out_dev, "gtn_mfma_selftest_i8: null output");
+    hipLaunchKernelGGL(k_mfma_selftest_i8, dim3(1), dim3(64), 0, as_stream(stream), out_dev);
+    GTN_CHECK_LAUNCH();
+    return GTN_OK;
+}
+
 extern "C" int gtn_mfma_selftest(double *out_dev, void *stream) {
     GTN_REQUIRE(out_dev, "gtn_mfma_selftest: null output");
     hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, as_stream(stream), out_dev);

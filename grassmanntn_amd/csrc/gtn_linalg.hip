@@ -847,6 +847,164 @@ __global__ void __launch_bounds__(256) k_rf_nt(RfArgs a) {
         }
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP64 products on the int8 matrix cores (Ozaki scheme).  An operand row r (length K) is scaled by
+// 2^-e_r, e_r = ilogb(max_k |x_rk|) + 1, and cut into kOzSlices signed 7-bit integer slices:
+// x_rk = 2^e_r (sum_s q_s 2^(-7 s) + rho), |q_s| <= 127, |rho| < 2^(-7 kOzSlices) (exact: scaling by a
+// power of two, then repeated multiplication by 128 and truncation, all exact in FP64).  A product
+// C = A B^T over two sliced operands is sum over slice pairs (s, t) of 2^(-7 (s + t)) (Q_s P_t^T): the
+// integer products are exact in int32 (v_mfma_i32_16x16x64_i8); pairs with s + t <= kOzSlices + 1
+// are kept (the dropped ones weigh <= 2^(-7 (kOzSlices + 2)) relative), their sums per diagonal
+// d = s + t stay exact in int32 for K <= kOzMaxK, and the diagonals are combined in FP64, smallest
+// first, then scaled by 2^(e_r + e'_c).  With eight slices (56 bits) the error of C_ij relative to
+// sum_k |A_ik| |B_jk| matches a native FP64 product's (1.5e-16 against 1.1e-16 on Gaussian 2048-deep
+// products; seven slices give 1.6e-14).
+// Slice planes: kOzSlices int8 matrices (rows_pad x K_pad, row-major, K contiguous), zero padded, and
+// one int32 exponent per row.
+// ---------------------------------------------------------------------------------------------
+constexpr int kOzSlices = 8;
+constexpr int kOzTile = 64;      // workgroup tile (rows and columns); also the row / K padding unit
+constexpr int64_t kOzMaxK = 16384;  // exact int32 diagonal sums: K * kOzSlices * 127^2 < 2^31
+static inline int64_t oz_pad(int64_t x) { return (x + kOzTile - 1) / kOzTile * kOzTile; }
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// one wavefront per row: row r of X (element (r, k) at X[r sr + k sk], k < K) -> slices and exponent;
+// rows r >= R (up to the grid's padded count) and columns k >= K get zero slices.  grid (Rpad / 4,
+// blocks), 256 lanes; blockIdx.y advances X by dX doubles, the planes by dP bytes, the exponents by dE.
+__global__ void __launch_bounds__(256) k_oz_slice(const double *X, int64_t sr, int64_t sk, int64_t R, int64_t K,
+                                                  int64_t Rpad, int64_t Kpad, int8_t *planes, int32_t *expo, int64_t dX,
+                                                  int64_t dP, int64_t dE) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= Rpad) return;
+    X += blockIdx.y * dX;
+    planes += blockIdx.y * dP;
+    expo += blockIdx.y * dE;
+    const bool live = r < R;
+    double mx = 0.0;
+    if (live)
+        for (int64_t k = lane; k < K; k += 64) mx = fmax(mx, fabs(X[r * sr + k * sk]));
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const int e = mx > 0.0 ? ilogb(mx) + 1 : 0;
+    if (lane == 0) expo[r] = e;
+    const int64_t plane = Rpad * Kpad;
+    for (int64_t k0 = 16 * lane; k0 < Kpad; k0 += 1024) {
+        int8_t q[kOzSlices][16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int64_t k = k0 + t;
+            double x = (live && k < K) ? ldexp(X[r * sr + k * sk], -e) : 0.0;
+#pragma unroll
+            for (int s = 0; s < kOzSlices; ++s) {
+                x *= 128.0;
+                const double qi = trunc(x);
+                q[s][t] = (int8_t)qi;
+                x -= qi;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kOzSlices; ++s) {
+            i32x4 v;
+            memcpy(&v, q[s], 16);
+            *(i32x4 *)(planes + s * plane + r * Kpad + k0) = v;
+        }
+    }
+}
+
+struct OzArgs {
+    const int8_t *A, *B;      // slice planes: A (rows_pad x K), B (cols_pad x K)
+    const int32_t *eA, *eB;   // row exponents of A and of B
+    double *C;                // C (rows x cols) at C[i ldc + j] (trans = 0) or C[j ldc + i] (trans = 1)
+    int64_t K, rows, cols, rows_pad, cols_pad, ldc;
+    int64_t dA, dB, dEA, dEB, dC;  // per-block strides (bytes for planes, ints for exponents, doubles for C)
+    int32_t nb, trans;
+};
+
+// C = A B^T, 64 x 64 tile per workgroup (4 waves, 2 x 2, 32 x 32 each); tiles of a block numbered
+// contiguously per XCD.  Per 64-deep K step: both operands' slice tiles in LDS (2 x 8 x 4 KiB), then
+// for every wave's 2 x 2 16x16 tiles and every kept slice pair one int8 MFMA into that pair's
+// diagonal accumulator.
+__global__ void __launch_bounds__(256) k_oz_nt(OzArgs a) {
+    const int64_t tr = a.rows_pad / kOzTile, tc = a.cols_pad / kOzTile, tiles = tr * tc, total = tiles * a.nb;
+    const int64_t per = (total + 7) / 8;
+    const int64_t t = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= total) return;
+    const int64_t blk = t / tiles, rem = t - blk * tiles, m0 = (rem / tc) * kOzTile, n0 = (rem % tc) * kOzTile;
+    const int8_t *A = a.A + blk * a.dA + m0 * a.K;
+    const int8_t *B = a.B + blk * a.dB + n0 * a.K;
+    const int64_t pa = a.rows_pad * a.K, pb = a.cols_pad * a.K;
+    // LDS rows of 64 bytes; the 16-byte chunk q of row r sits at chunk q ^ (r & 3), so the 16 rows a
+    // fragment read touches at one chunk index spread over four bank groups
+    __shared__ __attribute__((aligned(16))) int8_t As[kOzSlices][kOzTile][kOzTile];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[kOzSlices][kOzTile][kOzTile];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    i32x4 acc[kOzSlices][2][2];
+#pragma unroll
+    for (int d = 0; d < kOzSlices; ++d)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[d][i][j] = i32x4{0, 0, 0, 0};
+    // 16-byte chunk c of a slice tile: row c / 4, bytes 16 (c % 4); 256 chunks per slice tile
+    const int crow = threadIdx.x >> 2, ccol = 16 * (threadIdx.x & 3), cswz = 16 * ((threadIdx.x & 3) ^ (crow & 3));
+    for (int64_t k0 = 0; k0 < a.K; k0 += kOzTile) {
+        i32x4 ra[kOzSlices], rb[kOzSlices];
+#pragma unroll
+        for (int s = 0; s < kOzSlices; ++s) {
+            ra[s] = *(const i32x4 *)(A + s * pa + crow * a.K + k0 + ccol);
+            rb[s] = *(const i32x4 *)(B + s * pb + crow * a.K + k0 + ccol);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kOzSlices; ++s) {
+            *(i32x4 *)&As[s][crow][cswz] = ra[s];
+            *(i32x4 *)&Bs[s][crow][cswz] = rb[s];
+        }
+        __syncthreads();
+        i32x4 fa[kOzSlices][2], fb[kOzSlices][2];
+#pragma unroll
+        for (int s = 0; s < kOzSlices; ++s)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int ra_ = wm + 16 * i + lane % 16, rb_ = wn + 16 * i + lane % 16;
+                fa[s][i] = *(const i32x4 *)&As[s][ra_][16 * ((lane / 16) ^ (ra_ & 3))];
+                fb[s][i] = *(const i32x4 *)&Bs[s][rb_][16 * ((lane / 16) ^ (rb_ & 3))];
+            }
+#pragma unroll
+        for (int s = 0; s < kOzSlices; ++s)
+#pragma unroll
+            for (int u = 0; u < kOzSlices - s; ++u)  // pair (s + 1, u + 1), diagonal index s + u
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[s + u][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[s][i], fb[u][j], acc[s + u][i][j], 0, 0, 0);
+    }
+    // result register v of lane l: C[4 (l / 16) + v][l % 16] of each 16 x 16 tile
+    const int32_t *eA = a.eA + blk * a.dEA, *eB = a.eB + blk * a.dEB;
+    double *C = a.C + blk * a.dC;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t c = n0 + wn + 16 * j + lane % 16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t r = m0 + wm + 16 * i + 4 * (lane / 16) + v;
+                if (r >= a.rows || c >= a.cols) continue;
+                double sum = 0.0;
+#pragma unroll
+                for (int d = kOzSlices - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -7 * (d + 2));
+                const double val = ldexp(sum, eA[r] + eB[c]);
+                if (a.trans)
+                    C[c * a.ldc + r] = val;
+                else
+                    C[r * a.ldc + c] = val;
+            }
+        }
+}
+
 struct Blk {
     gtn_svd_block b;
     int64_t p;
@@ -857,8 +1015,18 @@ struct Blk {
     // (np x mp), Y^T (kRfTile x mp), Z^T (kRfTile x np), each hi then lo
     __bf16 *Mp, *MTp, *YTp, *ZTp;
     int64_t mp, np;
+    // int8 slices of the FP64-by-int8 range finder (oz_of(opts) and a large block): M (omp x onp),
+    // M^T (onp x omp), Y^T (opp x omp), Z^T (opp x onp), kOzSlices planes each, with row exponents
+    int8_t *Ms, *MTs, *YTs, *ZTs;
+    int32_t *eM, *eMT, *eYT, *eZT;
+    int64_t omp, onp, opp;
     int64_t bytes;
 };
+
+// range-finder products on int8 slices (SVD mode, blocks up to kOzMaxK on either side)
+static bool oz_of(const gtn_svd_opts &o, const gtn_svd_block &b) {
+    return o.mode == 0 && o.precision == GTN_PREC_F64_INT8 && oz_pad(b.m) <= kOzMaxK && oz_pad(b.n) <= kOzMaxK;
+}
 
 // reduced range-finder iterations on split-BF16 planes (SVD mode, BF16X3 arithmetic)
 static bool planes_of(const gtn_svd_opts &o) {
@@ -906,6 +1074,14 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
     // bf16 planes, in doubles: 2 (hi, lo) x 2 (M, M^T) x mp np / 4, 2 x kRfTile x (mp or np) / 4
     int64_t oMp = planes ? take(P.mp * P.np) : 0, oYT = planes ? take(kRfTile * P.mp / 2) : 0,
             oZT = planes ? take(kRfTile * P.np / 2) : 0;
+    const bool oz = !P.small && oz_of(o, b);
+    P.omp = oz_pad(b.m);
+    P.onp = oz_pad(b.n);
+    P.opp = oz_pad(p);
+    auto dbl = [](int64_t bytes) { return (bytes + 7) / 8; };
+    int64_t oMs = oz ? take(dbl(2 * kOzSlices * P.omp * P.onp)) : 0, oYs = oz ? take(dbl(kOzSlices * P.opp * P.omp)) : 0,
+            oZs = oz ? take(dbl(kOzSlices * P.opp * P.onp)) : 0,
+            oE = oz ? take(dbl(4 * (P.omp + P.onp + 2 * P.opp))) : 0;
     P.bytes = (off * 8 + 255) / 256 * 256;
     P.ws = ws;
     if (ws) {
@@ -928,6 +1104,14 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
         P.MTp = planes ? P.Mp + 2 * P.mp * P.np : nullptr;
         P.YTp = planes ? (__bf16 *)(ws + oYT) : nullptr;
         P.ZTp = planes ? (__bf16 *)(ws + oZT) : nullptr;
+        P.Ms = oz ? (int8_t *)(ws + oMs) : nullptr;
+        P.MTs = oz ? P.Ms + kOzSlices * P.omp * P.onp : nullptr;
+        P.YTs = oz ? (int8_t *)(ws + oYs) : nullptr;
+        P.ZTs = oz ? (int8_t *)(ws + oZs) : nullptr;
+        P.eM = oz ? (int32_t *)(ws + oE) : nullptr;
+        P.eMT = oz ? P.eM + P.omp : nullptr;
+        P.eYT = oz ? P.eMT + P.onp : nullptr;
+        P.eZT = oz ? P.eYT + P.opp : nullptr;
     }
 }
 
@@ -1278,6 +1462,51 @@ static int rf_iteration_planes(std::vector<Run> &rl, hipStream_t s) {
     return rf_y_from_zt(rl, s);
 }
 
+// int8 slices of every block's M (rows of M) and M^T (columns of M), once per call
+static int oz_slices_of_blocks(std::vector<Run> &rl, hipStream_t s) {
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        GTN_REQUIRE(f.Ms, "gtn_svd_blocks: no range-finder slices");
+        R8N(f.b.M, ext(f.b.m, f.b.n, f.b.lda), r.dM, r.n, "block");
+        R8N((const double *)f.Ms, (2 * kOzSlices * f.omp * f.onp + 7) / 8, r.dws, r.n, "slices of M");
+        R8N((const double *)f.eM, (4 * (f.omp + f.onp + 2 * f.opp) + 7) / 8, r.dws, r.n, "slice exponents");
+        LAUNCH(k_oz_slice, dim3((unsigned)(f.omp / 4), r.n), dim3(256), 0, s, f.b.M, f.b.lda, (int64_t)1, f.b.m, f.b.n,
+               f.omp, f.onp, f.Ms, f.eM, r.dM, 8 * r.dws, 2 * r.dws);
+        LAUNCH(k_oz_slice, dim3((unsigned)(f.onp / 4), r.n), dim3(256), 0, s, f.b.M, (int64_t)1, f.b.lda, f.b.n, f.b.m,
+               f.onp, f.omp, f.MTs, f.eMT, r.dM, 8 * r.dws, 2 * r.dws);
+    }
+    return GTN_OK;
+}
+
+static dim3 oz_grid(int64_t rows_pad, int64_t cols_pad, int nb) {
+    const int64_t total = rows_pad / kOzTile * (cols_pad / kOzTile) * nb;
+    return dim3((unsigned)((total + 7) / 8 * 8));
+}
+
+// out (rows x p FP64, ld p) = (A planes, rows) x (slices of the p columns of X (K x p FP64, ld p))
+static int oz_product(std::vector<Run> &rl, bool transposed_m, double *(Blk::*X), double *(Blk::*out), hipStream_t s) {
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        const int64_t K = transposed_m ? f.b.m : f.b.n, Kp = transposed_m ? f.omp : f.onp;
+        const int64_t rows = transposed_m ? f.b.n : f.b.m, rows_pad = transposed_m ? f.onp : f.omp;
+        int8_t *Bs = transposed_m ? f.YTs : f.ZTs;
+        int32_t *eB = transposed_m ? f.eYT : f.eZT;
+        R8N(f.*X, K * f.p, r.dws, r.n, "range-finder operand");
+        R8N(f.*out, rows * f.p, r.dws, r.n, "range-finder product");
+        LAUNCH(k_oz_slice, dim3((unsigned)(f.opp / 4), r.n), dim3(256), 0, s, (const double *)(f.*X), (int64_t)1, f.p,
+               f.p, K, f.opp, Kp, Bs, eB, r.dws, 8 * r.dws, 2 * r.dws);
+        OzArgs a;
+        a.A = transposed_m ? f.MTs : f.Ms, a.B = Bs;
+        a.eA = transposed_m ? f.eMT : f.eM, a.eB = eB;
+        a.C = f.*out;
+        a.K = Kp, a.rows = rows, a.cols = f.p, a.rows_pad = rows_pad, a.cols_pad = f.opp, a.ldc = f.p;
+        a.dA = a.dB = 8 * r.dws, a.dEA = a.dEB = 2 * r.dws, a.dC = r.dws;
+        a.nb = r.n, a.trans = 0;
+        LAUNCH(k_oz_nt, oz_grid(rows_pad, f.opp, r.n), dim3(256), 0, s, a);
+    }
+    return GTN_OK;
+}
+
 static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream_t s) {
     const int eig = o.mode == 1;
     std::vector<Blk *> small, large, all;
@@ -1327,6 +1556,12 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
             rc = rf_planes_of_blocks(rl, s);
             if (rc) return rc;
         }
+        bool oz = !eig && o.precision == GTN_PREC_F64_INT8;
+        for (Run &r : rl) oz = oz && r.f->Ms != nullptr;
+        if (oz) {
+            rc = oz_slices_of_blocks(rl, s);
+            if (rc) return rc;
+        }
         for (Run &r : rl) {
             Blk &f = *r.f;
             R8N(f.Z, f.b.n * f.p, r.dws, r.n, "gaussian");
@@ -1337,16 +1572,29 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
                 R8N((const double *)f.ZTp, kRfTile * f.np / 2, r.dws, r.n, "range-finder planes of Z");
                 LAUNCH(k_planes, dim3(kRfTile / 64, (unsigned)(f.np / 64), r.n), dim3(256), 0, s, (const double *)f.Z,
                        f.p, f.b.n, f.p, f.np, (int64_t)kRfTile, (__bf16 *)nullptr, f.ZTp, r.dws, 4 * r.dws);
-            } else {
+            } else if (!oz) {
                 g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));  // Y = M Z
             }
         }
-        rc = planes ? rf_y_from_zt(rl, s) : run(g, o.precision, s);
+        // FP64-by-int8 falls back to native FP64 where a block has no slices (beyond kOzMaxK)
+        const int gprec = o.precision == GTN_PREC_F64_INT8 ? GTN_PREC_F64 : o.precision;
+        rc = planes ? rf_y_from_zt(rl, s) : (oz ? oz_product(rl, false, &Blk::Z, &Blk::Y, s) : run(g, gprec, s));
         if (rc) return rc;
         rc = orthonormalise(rl, o.iters > 0 ? 1 : 2, s);
         if (rc) return rc;
         for (int it = 0; it < o.iters; ++it) {
-            const int prec = it >= o.iters - o.f64_tail ? GTN_PREC_F64 : o.precision;
+            const int prec = it >= o.iters - o.f64_tail ? GTN_PREC_F64 : (oz ? o.precision : gprec);
+            if (oz && prec != GTN_PREC_F64) {  // Z = M^T Y, Y = M Z on int8 slices
+                rc = oz_product(rl, true, &Blk::Y, &Blk::Z, s);
+                if (rc) return rc;
+                rc = oz_product(rl, false, &Blk::Z, &Blk::Y, s);
+                if (rc) return rc;
+                const int every = o.orth_every > 1 ? o.orth_every : 1;
+                rc = (it + 1 == o.iters) ? orthonormalise(rl, 2, s)
+                                         : ((it + 1) % every == 0 ? orthonormalise(rl, 1, s) : GTN_OK);
+                if (rc) return rc;
+                continue;
+            }
             if (planes && prec != GTN_PREC_F64) {
                 rc = rf_iteration_planes(rl, s);
                 if (rc) return rc;

@@ -44,6 +44,10 @@ extern "C" {
 #define GTN_PREC_BF16 4   /* operands rounded to BF16, one product, FP32 accumulation */
 #define GTN_PREC_BF16_SPLIT_A 5 /* A split hi+lo in BF16, B rounded to BF16, 2 products, FP32 accumulation
                                  * (BF16 weights times activations kept to ~16 bits) */
+#define GTN_PREC_F64_INT8 6 /* range finder of gtn_svd_blocks only: FP64 products on the int8 matrix cores
+                             * (each operand row scaled by a power of two and cut into eight 7-bit integer
+                             * slices, exact int32 slice products, FP64 recombination: the error of a
+                             * native FP64 product) */
 
 int gtn_version(void);
 
@@ -239,6 +243,13 @@ int64_t gtn_gemm_noalloc_max_scratch(void);
  * (768 doubles).  The layout the kernels use: out_dev[256 f + 4 lane + v] = D[row][lane % 16] with
  * row = lane / 16 + 4 v for FP64 and row = 4 (lane / 16) + v for FP32 and BF16. */
 int gtn_mfma_selftest(double *out_dev, void *stream);
+
+/* Device self-test of the int8 matrix-core form the FP64-by-int8 products use (no counterpart in the
+ * reference): one wavefront runs v_mfma_i32_16x16x64_i8 on A[i][k] = (i + 3k) % 7 - 3 and
+ * B[k][j] = (2j + k) % 5 - 2 (i, j < 16, k < 64), lane l feeding A[l % 16][16 (l / 16) + t] and
+ * B[16 (l / 16) + t][l % 16] as bytes t = 0..15, and stores its four result registers:
+ * out_dev[4 lane + v] (256 int32), expected to hold D[4 (lane / 16) + v][lane % 16]. */
+int gtn_mfma_selftest_i8(int32_t *out_dev, void *stream);
 
 /* ------------------------------------------------------------------------------------------------
  * Reductions and elementwise helpers (FP64 accumulation).

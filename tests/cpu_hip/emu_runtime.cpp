@@ -46,7 +46,7 @@ static std::unique_ptr<Bar[]> g_waves;
 static std::vector<std::vector<char>> g_wavebuf;  // per wave: 2 slots
 static char *g_dyn = nullptr;
 static size_t g_dyn_cap = 0, g_dyn_size = 0;
-static const size_t kSlot = 64 * 256;
+static const size_t kSlot = 64 * 512;  // per lane: up to 512 bytes (the MFMA emulation's operand slots)
 
 void *dyn_lds() { return g_dyn; }
 void sync_block() { g_block.wait(); }

@@ -60,7 +60,7 @@ extern Idx bdim, gdim;
 void *dyn_lds();
 void sync_block();
 void sync_wave();
-void *wave_buf(int slot);  // per-wave exchange area (64 lanes x 64 bytes per slot)
+void *wave_buf(int slot);  // per-wave exchange area (64 lanes x 512 bytes per slot)
 int lane();
 }  // namespace emu
 
@@ -142,7 +142,7 @@ inline TC emu_mfma16(const TA *a, const TA *b, TC c) {
     // C[l/16 + 4*v][l%16] for the FP64 16x16x4 form (ROW_INTERLEAVED: one row per register group;
     // composable_kernel's mfma_type<mfma_f64_16x16x4f64> has group_size 1, num_groups_per_blk 4)
     struct Slot {
-        double a[8], b[8], c[4];
+        double a[16], b[16], c[4];
     };
     Slot *buf = (Slot *)emu::wave_buf(1);
     int l = emu::lane();
@@ -181,6 +181,17 @@ inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf8 a, emu_bf8 b, emu_
         bb[t] = b[t];
     }
     return emu_mfma16<__bf16, emu_f4, 8>(aa, bb, c);
+}
+
+typedef int emu_i4 __attribute__((ext_vector_type(4)));
+// int8 16x16x64: each lane's 16 operand bytes are packed four to an int (little-endian)
+inline emu_i4 __builtin_amdgcn_mfma_i32_16x16x64_i8(emu_i4 a, emu_i4 b, emu_i4 c, int, int, int) {
+    signed char aa[16], bb[16];
+    for (int t = 0; t < 16; ++t) {
+        aa[t] = (signed char)((a[t / 4] >> (8 * (t % 4))) & 0xff);
+        bb[t] = (signed char)((b[t / 4] >> (8 * (t % 4))) & 0xff);
+    }
+    return emu_mfma16<signed char, emu_i4, 16>(aa, bb, c);
 }
 
 // ---- runtime API ----------------------------------------------------------------------------

@@ -7,9 +7,8 @@
   7.9e-10, 1.6e-11, 6.3e-13: the randomized range finder's 20 subspace iterations leave a
   spectrum-dependent residual; one point run alone gives the single-point path's result bit for
   bit) and the mixed
-  schedule (core.mixed_schedule: contractions FP64 for five steps, then FP32; the range finder in
-  split BF16 with three FP64 iterations at the end; Rayleigh-Ritz, normalisation and the log-norm sum
-  in FP64 throughout) within 1e-8;
+  schedule (core.mixed_schedule: contractions FP64 for five steps, then FP32; the SVD, normalisation
+  and the log-norm sum in FP64 throughout) within 1e-8;
 * graph replay against eager execution of the same steps (bit-identical) and the repeatability of
   replays from other inputs.
 """

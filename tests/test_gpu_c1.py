@@ -91,10 +91,8 @@ def test_c1_trg_fp64_matches_golden():
 def test_c1_trg_mixed_matches_golden(prec):
     """The mixed schedule (gtn.mixed_schedule): contractions FP64 in the first five steps (their
     normalisation errors weigh 2^-1 .. 2^-5 in ln Z per site), then FP32 MFMA (or split BF16, six
-    products, FP32 accumulation); at every step the range finder in split BF16 with its last two
-    iterations FP64, the Rayleigh-Ritz step, normalisation and log-norm sum FP64:
-    |d ln Z/site| <= 1e-8 after 20 steps (measured with three FP64 steps and an FP64 range finder:
-    2.1e-9 f32, 3.2e-10 bf16x6)."""
+    products, FP32 accumulation); the parity-block SVD, normalisation and log-norm sum FP64 at every
+    step: |d ln Z/site| <= 1e-8 after 20 steps (measured 2.0e-9 f32, 1.2e-9 bf16x6)."""
     g = golden()
     res, _ = run_trg(O.site_tensor(64, g["seed"]), 64, 20, lambda i: gtn.mixed_schedule(i, prec))
     ref = complex(g["lnz_per_site_real"][-1], g["lnz_per_site_imag"][-1])
@@ -128,6 +126,18 @@ def test_c1_trg_split_bf16_range_finder_matches_fp64():
     err = abs(res[-1][1] - ref)
     print("C1 f64 contractions, split-BF16 range finder: |dlnZ/site| = %.3e" % err)
     assert err <= 1e-9, err
+
+
+def test_c1_trg_int8_range_finder_matches_golden():
+    """The range finder's products in FP64 on the int8 matrix cores (eight 7-bit slices per operand
+    row, exact integer products, gtn.precision('f64', 'f64_int8')), every contraction FP64: ln Z per
+    site after 20 steps within the FP64 path's bound of the golden."""
+    g = golden()
+    ref = complex(g["lnz_per_site_real"][-1], g["lnz_per_site_imag"][-1])
+    res, _ = run_trg(O.site_tensor(64, g["seed"]), 64, 20, lambda i: ("f64", "f64_int8"))
+    err = abs(res[-1][1] - ref)
+    print("C1 f64 contractions, FP64-by-int8 range finder: |dlnZ/site| = %.3e" % err)
+    assert err <= 2e-10, err
 
 
 def test_c1_einsum_ijkl_klmn():

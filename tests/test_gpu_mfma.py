@@ -72,3 +72,23 @@ def test_gemm_split_reduction(prec, tol):
         scale = np.abs(ref).max()
         np.testing.assert_allclose(got[:, :, N:], ref[:, :, N:], rtol=0, atol=0)  # padding untouched
         assert np.abs(got - ref).max() <= tol * scale, (prec, np.abs(got - ref).max() / scale)
+
+
+def test_int8_mfma_lane_layout():
+    """v_mfma_i32_16x16x64_i8 (the FP64-by-int8 range finder's product): lane l feeds
+    A[l % 16][16 (l / 16) + t] and B[16 (l / 16) + t][l % 16] (bytes t = 0..15) and receives
+    D[4 (l / 16) + v][l % 16] in result register v."""
+    import ctypes as C
+    from grassmanntn_amd import _lib as L
+    from grassmanntn_amd._device import DeviceArray
+    out = DeviceArray.empty((256,), np.int32)
+    L.check(L.lib().gtn_mfma_selftest_i8(C.c_void_p(out.data_ptr()), L.stream_ptr()), "gtn_mfma_selftest_i8")
+    got = out.numpy().reshape(64, 4)
+    i, k, j = np.arange(16), np.arange(64), np.arange(16)
+    A = (i[:, None] + 3 * k[None, :]) % 7 - 3
+    B = (2 * j[None, :] + k[:, None]) % 5 - 2
+    D = A @ B
+    exp = np.array([[D[4 * (l // 16) + v][l % 16] for v in range(4)] for l in range(64)])
+    if not np.array_equal(got, exp):
+        print("device result registers (lane x register):\n", got)
+    np.testing.assert_array_equal(got, exp)
