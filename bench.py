@@ -353,6 +353,8 @@ def main():
                     help="only run the copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--rf-gemm-only", action="store_true",
                     help="only time the range finder's FP64 GEMM shape (the roofline kernel; counter runs)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only run the plain FP64 GPU path (profiles of the baseline)")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="f32", help="contraction arithmetic of the mixed path")
@@ -390,6 +392,10 @@ def main():
         return
     if args.rf_gemm_only:
         print(json.dumps({"roofline": rf_gemm_roofline(args.batch)}))
+        return
+    if args.plain_only:
+        plain = run_path(gtn, "f64", "f64", args.steps, args.warmup, site_tensor(D, SEED))
+        print(json.dumps({"fp64_steps_per_s": args.steps / (plain["gpu_ms"] / 1000.0), "lnz": str(plain["lnz"])}))
         return
     if args.svd_iters:
         gtn.settings.svd_iters = args.svd_iters
