@@ -7,10 +7,11 @@ Grassmann-even site tensors (entries N(0,1) times exp(-0.5 * sum of leg popcount
 zeroed, max|T| = 1, parity-preserving legs, statistics (1,1,-1,-1); seeds 1..P) coarse-grained
 together for --steps (20) steps, every step one replay of a recorded HIP graph; arithmetic
 (gtn.mixed_schedule): contractions FP64 in the first five steps, then FP32 MFMA with FP32
-accumulation; the parity-block SVD (range finder, Rayleigh-Ritz, vectors), the normalisation and the
-log-norm sum FP64 at every step.  Steps/s counts (point, step) pairs.  Beside it: the same batch with
-other range-finder arithmetic (split BF16 with FP64 closing iterations; FP64 on int8 slices), each
-with its speed-up and ln Z error, the plain FP64 GPU path (one tensor, the eager gtn.trg loop)
+accumulation; the parity-block SVD (range finder iterating on the FP64 Gram matrix, Rayleigh-Ritz,
+vectors), the normalisation and the log-norm sum FP64 at every step.  Steps/s counts (point, step)
+pairs.  Beside it: the same batch with other range-finder arithmetic (split BF16 with FP64 closing
+iterations; alternating FP64 products; FP64 on int8 slices), each with its speed-up and ln Z error,
+the plain FP64 GPU path (one tensor, the eager gtn.trg loop, the same FP64 SVD)
 and the FP64 batched path (same batch, all FP64), ln Z per site of every point with an oracle golden
 against that golden, the roofline of the step's dominant kernel (the range finder's FP64 GEMM, timed
 with HIP events on the same shape), the coupling sweep (512 points, leg-dim-32 generators, chi from
@@ -100,7 +101,7 @@ def run_path(gtn, prec, svd_prec, steps, warmup, T0, fp64_steps=0):
     from grassmanntn_amd.gauge2d import trg_counters
 
     def step(T, i):
-        p = ("f64", "f64") if i < fp64_steps else (prec, svd_prec)
+        p = ("f64", svd_prec) if i < fp64_steps else (prec, svd_prec)
         if VERBOSE:
             print("step %d (%s contractions, %s range finder)" % (i, p[0], p[1]), file=sys.stderr, flush=True)
         with gtn.precision(*p):
@@ -394,7 +395,7 @@ def main():
         print(json.dumps({"roofline": rf_gemm_roofline(args.batch)}))
         return
     if args.plain_only:
-        plain = run_path(gtn, "f64", "f64", args.steps, args.warmup, site_tensor(D, SEED))
+        plain = run_path(gtn, "f64", gtn.settings.svd_precision, args.steps, args.warmup, site_tensor(D, SEED))
         print(json.dumps({"fp64_steps_per_s": args.steps / (plain["gpu_ms"] / 1000.0), "lnz": str(plain["lnz"])}))
         return
     if args.svd_iters:
@@ -405,7 +406,7 @@ def main():
         gtn.settings.svd_oversample = args.svd_oversample
     P, K = args.batch, args.steps
     prec = args.precision
-    svd_prec = args.svd_precision or gtn.core.MIXED_SVD
+    svd_prec = args.svd_precision or gtn.settings.svd_precision
     mixed_sched = lambda i: gtn.mixed_schedule(i, prec, svd=svd_prec, fp64_steps=args.fp64_steps)
     T0s = np.stack([site_tensor(D, SEED + s) for s in range(P)])
     golden = golden_lnz(K)
@@ -447,6 +448,7 @@ def main():
         out["range_finder_variants"] = {}
         for spec, what in (("bf16x3:3", "split BF16 (three BF16 products, FP32 accumulation) except the last "
                                         "three iterations, FP64; Rayleigh-Ritz FP64"),
+                           ("f64", "FP64, alternating products M^T Y and M Z per iteration"),
                            ("f64_int8", "FP64 products on the int8 matrix cores (eight 7-bit slices per operand "
                                         "row, exact integer products)")):
             if spec == svd_prec:
@@ -463,14 +465,14 @@ def main():
             out["range_finder_variants"][spec] = v
     if not args.no_fp64:
         progress("FP64 batched path")
-        f64b = batched_path(gtn, P, K, args.warmup, lambda i: ("f64", "f64"), T0s)
+        f64b = batched_path(gtn, P, K, args.warmup, lambda i: ("f64", gtn.settings.svd_precision), T0s)
         out["fp64_batched_steps_per_s"] = P * K / (f64b["gpu_ms"] / 1000.0)
         out["speedup_mixed_vs_fp64_batched"] = value / out["fp64_batched_steps_per_s"]
         if golden:
             out["dlnz_fp64_batched_vs_oracle"] = max(abs(f64b["lnz"][s - SEED] - g) for s, g in golden.items()
                                                      if s - SEED < P)
         progress("plain FP64 path")
-        plain = run_path(gtn, "f64", "f64", K, args.warmup, T0s[0])
+        plain = run_path(gtn, "f64", gtn.settings.svd_precision, K, args.warmup, T0s[0])
         out["fp64_steps_per_s"] = K / (plain["gpu_ms"] / 1000.0)
         out["speedup_mixed_vs_fp64"] = value / out["fp64_steps_per_s"]
         for v in out.get("range_finder_variants", {}).values():

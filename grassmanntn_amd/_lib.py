@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libgtn_hip.so")
 
 GTN_F64, GTN_F32, GTN_C128, GTN_I8 = 0, 1, 2, 3
 GTN_ACT_NONE, GTN_ACT_GELU = 0, 1
-PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16": 4, "bf16_split_a": 5, "f64_int8": 6}
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16": 4, "bf16_split_a": 5, "f64_int8": 6, "f64_gram": 7}
 MAX_AX, MAX_COMP, MAX_BITS = 10, 16, 10
 
 

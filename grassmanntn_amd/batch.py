@@ -272,7 +272,7 @@ def _new_pool():
 class BatchedTRG:
     """TRG of P coupling points of equal leg dimension together.
 
-    schedule(i) -> (contraction, svd) arithmetic of step i (default all FP64; see
+    schedule(i) -> (contraction, svd) arithmetic of step i (default FP64 with the library's SVD; see
     core.mixed_schedule).  With graphs=True each distinct (input shape, arithmetic) step is recorded
     once as a HIP graph (after one eager warm-up step that fills the library's plan and table caches)
     and every step of a run is one graph launch; graphs=False runs the same steps eagerly.
@@ -284,7 +284,7 @@ class BatchedTRG:
 
     def __init__(self, P, D, dcut, steps, schedule=None, graphs=True):
         self.P, self.D, self.dcut, self.steps = P, D, dcut, steps
-        self.schedule = schedule or (lambda i: ("f64", "f64"))
+        self.schedule = schedule or (lambda i: ("f64", settings.svd_precision))
         self.use_graphs = graphs
         self.Dn = _bond(min(D * D // 2, int(dcut / 2)))  # bond after the first step
         self.x0 = DeviceArray.zeros((P, D ** 4))

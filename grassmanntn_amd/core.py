@@ -32,7 +32,9 @@ def _pkg_flag(name, default):
 # --------------------------------------------------------------------------------------------------
 class _Settings:
     precision = "f64"       # GEMM arithmetic of einsum: f64 | f32 | bf16x3 | bf16x6
-    svd_precision = "f64"   # range-finder GEMMs of the block SVD
+    # range-finder arithmetic of the block SVD (svd_spec): FP64 subspace iteration on the Gram matrix
+    # M M^T, one FP64 product per iteration (the alternating M^T Y, M Z products: "f64")
+    svd_precision = "f64_gram"
     svd_oversample = 96
     svd_iters = 20          # subspace iterations: error ~ (sigma_{p+1} / sigma_k)^(2 iters)
     # CholeskyQR of the basis every this many iterations.  Every iteration: between two
@@ -59,6 +61,9 @@ def svd_iters_for(k):
     return int(math.ceil(settings.svd_iters * max(1.0, r * r)))
 
 
+SVD_ONLY = ("f64_int8", "f64_gram")  # range-finder arithmetic that the contraction GEMM does not have
+
+
 def svd_spec(spec):
     """(range-finder GEMM precision code, FP64 tail) of a decomposition arithmetic: a precision name
     ("f64", "f32", "bf16x3", ...) runs every range-finder iteration in that arithmetic;
@@ -78,8 +83,8 @@ class precision:
     def __init__(self, gemm="f64", svd=None):
         if svd is not None:
             svd_spec(svd)
-        if gemm not in L.PREC or gemm == "f64_int8":
-            raise ValueError("precision must be one of %s" % [k for k in L.PREC if k != "f64_int8"])
+        if gemm not in L.PREC or gemm in SVD_ONLY:
+            raise ValueError("precision must be one of %s" % [k for k in L.PREC if k not in SVD_ONLY])
         self.gemm, self.svd = gemm, svd
 
     def __enter__(self):
@@ -94,20 +99,20 @@ class precision:
 
 
 MIXED_FP64_STEPS = 5
-MIXED_SVD = "f64"
 
 
-def mixed_schedule(step, contraction="f32", svd=MIXED_SVD, fp64_steps=MIXED_FP64_STEPS):
+def mixed_schedule(step, contraction="f32", svd=None, fp64_steps=MIXED_FP64_STEPS):
     """(contraction, svd) arithmetic of coarse-graining step `step` (0-based) in the mixed-precision
     schedule.  Contractions: FP64 in the first `fp64_steps` steps, then `contraction`.  The FP64
     leading steps are there because ln Z per site weighs the normalisation of step n by 2^-n
     (example.py:191-193): FP32 rounding (relative ~1e-7 in a step's norm) costs ~1e-8 when it starts
     at step 4 and ~4e-9 from step 6 on (measured at leg dims 16 and 64).  Decompositions: `svd` (see
-    svd_spec), by default FP64 throughout; the normalisation and the log-norm sum are FP64.  Other
-    range-finder arithmetic measured at leg dim 64, chi 64 with FP64 contractions, largest
-    |d ln Z/site| over four seeds: split BF16 with its last three iterations FP64 ('bf16x3:3')
-    2.6e-10 (two: 1.8e-9, six: 2.4e-11); FP64 on int8 slices ('f64_int8') as the native FP64."""
-    return ("f64" if step < fp64_steps else contraction, svd)
+    svd_spec; default settings.svd_precision, FP64 throughout); the normalisation and the log-norm
+    sum are FP64.  Other range-finder arithmetic measured at leg dim 64, chi 64 with FP64
+    contractions, largest |d ln Z/site| over four seeds: split BF16 with its last three iterations
+    FP64 ('bf16x3:3') 2.6e-10 (two: 1.8e-9, six: 2.4e-11); FP64 on int8 slices ('f64_int8') and the
+    alternating FP64 products ('f64') as the Gram iteration."""
+    return ("f64" if step < fp64_steps else contraction, svd if svd is not None else settings.svd_precision)
 
 
 def _to_device(x):
@@ -902,10 +907,8 @@ def _einsum2(plan, A, B, first, ignore):
     dims = plan.dims
     setA, setB, setO = set(sA), set(sB), set(out)
     uA, uB = list(dict.fromkeys(sA)), list(dict.fromkeys(sB))
-    dA, dB = _source_dtype(A), _source_dtype(B)
-    if np.dtype(dA).kind == "c" or np.dtype(dB).kind == "c":
-        raise NotImplementedError("complex contractions are not provided by this build")
-    if dA != dB:
+    dA, dB = np.dtype(_source_dtype(A)), np.dtype(_source_dtype(B))
+    if not all(d in (np.dtype(np.float64), np.dtype(np.complex128)) for d in (dA, dB)) and dA != dB:
         raise NotImplementedError("operands of different dtypes")
     isF = lambda c: _fermionic(plan, c)
     batch = [c for c in uA if c in setB and c in setO]
@@ -1047,8 +1050,73 @@ def _einsum2(plan, A, B, first, ignore):
     return res
 
 
+def _planes_of_complex(X):
+    """(2, n) FP64 device array: the real then the imaginary parts of complex128 X (n values), one
+    gather over X's storage read as 2n doubles."""
+    n = X.numel()
+    raw = DeviceArray(X.storage, X.offset, (2 * n,), np.float64)
+    P = Perm()
+    i_el, i_part = P.add_in(n, 2), P.add_in(2, 1)
+    v_part, v_el = P.add_virt(2), P.add_virt(n)
+    P.add_comp(v_part, 2, [(i_part, 1)])
+    P.add_comp(v_el, n, [(i_el, 1)])
+    P.add_out(2, [(v_part, 1)])
+    P.add_out(n, [(v_el, 1)])
+    return P.run(raw)
+
+
+def _complex_of_planes(R, n):
+    """complex128 array of n values from the (2, n) FP64 planes R (real, imaginary)."""
+    out = DeviceArray.empty((n,), np.complex128)
+    raw = DeviceArray(out.storage, out.offset, (2 * n,), np.float64)
+    P = Perm()
+    i_part, i_el = P.add_in(2, n), P.add_in(n, 1)
+    v_el, v_part = P.add_virt(n), P.add_virt(2)
+    P.add_comp(v_el, n, [(i_el, 1)])
+    P.add_comp(v_part, 2, [(i_part, 1)])
+    P.add_out(n, [(v_el, 1)])
+    P.add_out(2, [(v_part, 1)])
+    P.run(R, dst=raw)
+    return out
+
+
+def _contract_blocks_complex(Ag, Bg, nb, nP, M, N, K):
+    """C[b][p] = A[b][p] B[b][p] with complex128 blocks (one operand may be real FP64): the complex
+    operands are gathered into real and imaginary planes, combined by real GEMMs -- Re C = Re A Re B
+    - Im A Im B, Im C = Re A Im B + Im A Re B, the terms of a real operand's zero imaginary part left
+    out -- in two launches (the second accumulates, beta = 1), and interleaved back."""
+    ca, cb = Ag.dtype.kind == "c", Bg.dtype.kind == "c"
+    na, nbn = Ag.numel(), Bg.numel()
+    Ap = _planes_of_complex(Ag) if ca else Ag
+    Bp = _planes_of_complex(Bg) if cb else Bg
+    nc = nb * nP * M * N
+    Cp = DeviceArray.empty((2, nc))
+    es = 8
+
+    def desc(imag_a, imag_b, imag_c, alpha, beta, p):
+        a = Ap.data_ptr() + es * ((na if imag_a else 0) + p * M * K)
+        b = Bp.data_ptr() + es * ((nbn if imag_b else 0) + p * K * N)
+        c = Cp.data_ptr() + es * ((nc if imag_c else 0) + p * M * N)
+        return _gemm_desc(a, K, 1, b, N, 1, c, N, 1, M, N, K, batch=nb, sab=nP * M * K, sbb=nP * K * N,
+                          scb=nP * M * N, alpha=alpha, beta=beta)
+
+    first, second = [], []
+    for p in range(nP):
+        first.append(desc(False, False, False, 1.0, 0.0, p))
+        first.append(desc(False, True, True, 1.0, 0.0, p) if cb else desc(True, False, True, 1.0, 0.0, p))
+        if ca and cb:
+            second.append(desc(True, True, False, -1.0, 1.0, p))
+            second.append(desc(True, False, True, 1.0, 1.0, p))
+    _gemm(first, L.GTN_F64, settings.precision)
+    if second:
+        _gemm(second, L.GTN_F64, settings.precision)
+    return _complex_of_planes(Cp, nc)
+
+
 def _contract_blocks(Ag, Bg, nb, nP, M, N, K):
     """C[b][p] = A[b][p] (M x K) * B[b][p] (K x N) for every batch b and parity block p."""
+    if Ag.dtype.kind == "c" or Bg.dtype.kind == "c":
+        return _contract_blocks_complex(Ag, Bg, nb, nP, M, N, K)
     dt = L.dtype_code(Ag)
     prec = settings.precision if Ag.dtype == np.float64 else "f32"
     Cg = DeviceArray.empty(nb * nP * M * N, Ag.dtype)

@@ -91,6 +91,26 @@ __global__ void k_transpose2d(const double *src, int64_t lds, double *dst, int64
         dst[c * ldd + r] = src[r * lds + c];
     }
 }
+// G[r][c] = G[c][r] for the entries whose row super-block (r / S) lies below their column's
+// (c / S): the part of a symmetric n x n matrix (ld) that a product over its upper block triangle
+// left out.  64 x 64 tiles through LDS; grid (n / 64 rounded up, same, blocks), blockIdx.z advancing
+// G by dG.
+__global__ void __launch_bounds__(256) k_mirror_lower(double *G, int64_t n, int64_t ld, int64_t S, int64_t dG) {
+    __shared__ double tile[64][65];
+    G += blockIdx.z * dG;
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    if (r0 / S <= (c0 + 63) / S) return;  // the tile is not wholly below the computed block triangle
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {  // read the mirror tile (rows c0.., columns r0..) row by row
+        const int64_t rr = c0 + i, cc = r0 + tx;
+        tile[i][tx] = (rr < n && cc < n) ? G[rr * ld + cc] : 0.0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t rr = r0 + i, cc = c0 + tx;
+        if (rr < n && cc < n) G[rr * ld + cc] = tile[tx][i];
+    }
+}
 __global__ void k_fill(double *x, int64_t rows, int64_t cols, int64_t ld, double v, int64_t dx) {
     x += blockIdx.y * dx;
     int64_t total = rows * cols;
@@ -1020,8 +1040,14 @@ struct Blk {
     int8_t *Ms, *MTs, *YTs, *ZTs;
     int32_t *eM, *eMT, *eYT, *eZT;
     int64_t omp, onp, opp;
+    double *Gm;  // m x m Gram matrix M M^T of the Gram-iteration range finder (gram_of(opts))
     int64_t bytes;
 };
+
+// range finder iterating on the FP64 Gram matrix M M^T (SVD mode, m <= n)
+static bool gram_of(const gtn_svd_opts &o, const gtn_svd_block &b) {
+    return o.mode == 0 && o.precision == GTN_PREC_F64_GRAM && b.m <= b.n;
+}
 
 // range-finder products on int8 slices (SVD mode, blocks up to kOzMaxK on either side)
 static bool oz_of(const gtn_svd_opts &o, const gtn_svd_block &b) {
@@ -1082,6 +1108,8 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
     int64_t oMs = oz ? take(dbl(2 * kOzSlices * P.omp * P.onp)) : 0, oYs = oz ? take(dbl(kOzSlices * P.opp * P.omp)) : 0,
             oZs = oz ? take(dbl(kOzSlices * P.opp * P.onp)) : 0,
             oE = oz ? take(dbl(4 * (P.omp + P.onp + 2 * P.opp))) : 0;
+    const bool gram = !P.small && gram_of(o, b);
+    int64_t oGm = gram ? take(b.m * b.m) : 0;
     P.bytes = (off * 8 + 255) / 256 * 256;
     P.ws = ws;
     if (ws) {
@@ -1112,6 +1140,7 @@ static void plan_block(const gtn_svd_block &b, const gtn_svd_opts &o, Blk &P, do
         P.eMT = oz ? P.eM + P.omp : nullptr;
         P.eYT = oz ? P.eMT + P.onp : nullptr;
         P.eZT = oz ? P.eYT + P.opp : nullptr;
+        P.Gm = gram ? ws + oGm : nullptr;
     }
 }
 
@@ -1507,6 +1536,44 @@ static int oz_product(std::vector<Run> &rl, bool transposed_m, double *(Blk::*X)
     return GTN_OK;
 }
 
+// the Gram matrices G = M M^T of every block (m x m, FP64): products over the upper block triangle
+// in super-rows of S rows (one grouped launch), then the lower part mirrored
+static int64_t gram_super_rows(int64_t m) {
+    int64_t S = 256;
+    while ((m + S - 1) / S > 12) S += 128;  // at most 12 groups per launch
+    return S;
+}
+static int gram_of_blocks(std::vector<Run> &rl, hipStream_t s) {
+    std::vector<gtn_gemm_desc> g;
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        GTN_REQUIRE(f.Gm, "gtn_svd_blocks: no Gram matrix");
+        const int64_t m = f.b.m, S = gram_super_rows(m);
+        R8N(f.Gm, m * m, r.dws, r.n, "Gram matrix");
+        R8N(f.b.M, ext(f.b.m, f.b.n, f.b.lda), r.dM, r.n, "block");
+        if (g.size() + (size_t)((m + S - 1) / S) > 12) {  // one launch takes at most 12 groups
+            int rc = run(g, GTN_PREC_F64, s);
+            if (rc) return rc;
+            g.clear();
+        }
+        for (int64_t r0 = 0; r0 < m; r0 += S) {
+            const int64_t rows = m - r0 < S ? m - r0 : S;
+            // G[r0.., r0..] = M[r0.., :] M[r0.., :]^T
+            g.push_back(gdr(f.b.M + r0 * f.b.lda, f.b.lda, 1, f.b.M + r0 * f.b.lda, 1, f.b.lda, f.Gm + r0 * m + r0, m, rows,
+                            m - r0, f.b.n, r.n, r.dM, r.dM, r.dws));
+        }
+    }
+    int rc = run(g, GTN_PREC_F64, s);
+    if (rc) return rc;
+    for (Run &r : rl) {
+        Blk &f = *r.f;
+        const int64_t m = f.b.m, t = (m + 63) / 64;
+        GTN_REQUIRE(t <= 65535, "gtn_svd_blocks: Gram matrix too large");
+        LAUNCH(k_mirror_lower, dim3((unsigned)t, (unsigned)t, r.n), dim3(256), 0, s, f.Gm, m, m, gram_super_rows(m), r.dws);
+    }
+    return GTN_OK;
+}
+
 static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream_t s) {
     const int eig = o.mode == 1;
     std::vector<Blk *> small, large, all;
@@ -1562,6 +1629,12 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
             rc = oz_slices_of_blocks(rl, s);
             if (rc) return rc;
         }
+        bool gram = !eig && o.precision == GTN_PREC_F64_GRAM;
+        for (Run &r : rl) gram = gram && r.f->Gm != nullptr;
+        if (gram) {
+            rc = gram_of_blocks(rl, s);
+            if (rc) return rc;
+        }
         for (Run &r : rl) {
             Blk &f = *r.f;
             R8N(f.Z, f.b.n * f.p, r.dws, r.n, "gaussian");
@@ -1576,14 +1649,32 @@ static int run_blocks(std::vector<Blk> &blocks, const gtn_svd_opts &o, hipStream
                 g.push_back(gdr(f.b.M, f.b.lda, 1, f.Z, f.p, 1, f.Y, f.p, f.b.m, f.p, f.b.n, r.n, r.dM, r.dws, r.dws));  // Y = M Z
             }
         }
-        // FP64-by-int8 falls back to native FP64 where a block has no slices (beyond kOzMaxK)
-        const int gprec = o.precision == GTN_PREC_F64_INT8 ? GTN_PREC_F64 : o.precision;
+        // FP64-by-int8 and the Gram iteration fall back to the alternating FP64 products where a
+        // block has no slices (beyond kOzMaxK) or no Gram matrix (m > n); their first product is FP64
+        const int gprec = (o.precision == GTN_PREC_F64_INT8 || o.precision == GTN_PREC_F64_GRAM) ? GTN_PREC_F64 : o.precision;
         rc = planes ? rf_y_from_zt(rl, s) : (oz ? oz_product(rl, false, &Blk::Z, &Blk::Y, s) : run(g, gprec, s));
         if (rc) return rc;
         rc = orthonormalise(rl, o.iters > 0 ? 1 : 2, s);
         if (rc) return rc;
         for (int it = 0; it < o.iters; ++it) {
             const int prec = it >= o.iters - o.f64_tail ? GTN_PREC_F64 : (oz ? o.precision : gprec);
+            if (gram) {  // Y = G Y (into Z, which then becomes the basis)
+                g.clear();
+                for (Run &r : rl) {
+                    Blk &f = *r.f;
+                    R8N(f.Z, f.b.m * f.p, r.dws, r.n, "Gram iterate");
+                    g.push_back(gdr(f.Gm, f.b.m, 1, f.Y, f.p, 1, f.Z, f.p, f.b.m, f.p, f.b.m, r.n, r.dws, r.dws, r.dws));
+                }
+                rc = run(g, GTN_PREC_F64, s);
+                if (rc) return rc;
+                for (Run &r : rl)
+                    for (Blk *b : r.mem) std::swap(b->Y, b->Z);
+                const int every = o.orth_every > 1 ? o.orth_every : 1;
+                rc = (it + 1 == o.iters) ? orthonormalise(rl, 2, s)
+                                         : ((it + 1) % every == 0 ? orthonormalise(rl, 1, s) : GTN_OK);
+                if (rc) return rc;
+                continue;
+            }
             if (oz && prec != GTN_PREC_F64) {  // Z = M^T Y, Y = M Z on int8 slices
                 rc = oz_product(rl, true, &Blk::Y, &Blk::Z, s);
                 if (rc) return rc;

@@ -48,6 +48,10 @@ extern "C" {
                              * (each operand row scaled by a power of two and cut into eight 7-bit integer
                              * slices, exact int32 slice products, FP64 recombination: the error of a
                              * native FP64 product) */
+#define GTN_PREC_F64_GRAM 7 /* range finder of gtn_svd_blocks only: FP64 subspace iteration on the Gram
+                             * matrix G = M M^T (formed once per call from its upper block triangle), one
+                             * product G Y per iteration instead of M^T Y and M Z; blocks with m > n
+                             * use the alternating FP64 products */
 
 int gtn_version(void);
 

@@ -140,6 +140,18 @@ def test_c1_trg_int8_range_finder_matches_golden():
     assert err <= 2e-10, err
 
 
+def test_c1_trg_gram_range_finder_matches_golden():
+    """The range finder iterating on the FP64 Gram matrix M M^T (gtn.precision('f64', 'f64_gram'): one
+    FP64 product per iteration), every contraction FP64: ln Z per site after 20 steps within the
+    FP64 path's bound of the golden."""
+    g = golden()
+    ref = complex(g["lnz_per_site_real"][-1], g["lnz_per_site_imag"][-1])
+    res, _ = run_trg(O.site_tensor(64, g["seed"]), 64, 20, lambda i: ("f64", "f64_gram"))
+    err = abs(res[-1][1] - ref)
+    print("C1 f64 contractions, Gram-iteration range finder: |dlnZ/site| = %.3e" % err)
+    assert err <= 2e-10, err
+
+
 def test_c1_einsum_ijkl_klmn():
     """'ijkl,klmn->ijmn' on two seeded 64^4 Grassmann-even tensors (a 4096 x 4096 x 4096 contraction
     whose nonzero parity blocks are two 2048^3 GEMMs): FP64 result within 1e-12 of the oracle's
