@@ -364,7 +364,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5,
                     help="leading TRG steps of the mixed path run in FP64 (their normalisation errors "
                          "weigh 2^-n in ln Z per site)")
-    ap.add_argument("--batch", type=int, default=16, help="points coarse-grained together in the headline path")
+    ap.add_argument("--batch", type=int, default=64, help="points coarse-grained together in the headline path")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 512-point coupling sweep")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned surrogate")

@@ -99,6 +99,59 @@ __device__ __forceinline__ void load_panel(const T *p, int64_t sr, int64_t sk, i
         reg[s] = ok ? v : 0.0;
     }
 }
+// Range-checked panel walk: load_panel at every panel.
+template <typename T, int ROWS, int BK>
+struct CheckedPanels {
+    const T *p;
+    int64_t sr, sk;
+    int R0, R, K;
+    __device__ __forceinline__ void load(int k0, double (&reg)[ROWS * BK / 256]) const {
+        load_panel<T, ROWS, BK>(p, sr, sk, R0, k0, R, K, reg);
+    }
+};
+
+// Panel walk over an operand whose ROWS x K slab (rows R0.., all of the reduction) lies wholly in
+// range, every byte of it within 2^31 of the slab's first element.  The lane -> element map of
+// load_panel is affine in the slot (256 is a multiple of BK and of ROWS): slot s holds element
+// (r + s dr, k + s dk), so each slot is one buffer load from a resource at the slab's origin, this
+// lane's 32-bit byte offset (advanced by BK elements of the reduction per panel) and a uniform
+// per-slot byte offset -- no per-slot index arithmetic or range test.
+template <typename T, int ROWS, int BK>
+struct InnerPanels {
+    static_assert(256 % BK == 0 && 256 % ROWS == 0, "the slot map needs 256 to be a multiple of BK and ROWS");
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t voff0, kbytes, sbytes;
+    __device__ __forceinline__ InnerPanels(const T *p, int64_t sr, int64_t sk, int R0) {
+        const bool kc = (sk == 1);
+        const int t = threadIdx.x;
+        const int r = kc ? t / BK : t % ROWS, k = kc ? t % BK : t / ROWS;
+        // raw buffer (stride 0), num_records in bytes; word 3 = 0x00020000 for the gfx9 family.  The
+        // origin and the strides are the same in every lane: readfirstlane states it, so that the
+        // resource and the slot offsets live in scalar registers.
+        const uint64_t o = (uint64_t)(p + (int64_t)R0 * sr);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)o), hi = __builtin_amdgcn_readfirstlane((uint32_t)(o >> 32));
+        rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0, 0x7fffffff, 0x00020000);
+        voff0 = (uint32_t)(((int64_t)r * sr + (int64_t)k * sk) * (int64_t)sizeof(T));
+        kbytes = __builtin_amdgcn_readfirstlane((uint32_t)(sk * (int64_t)sizeof(T)));
+        sbytes = __builtin_amdgcn_readfirstlane((uint32_t)((kc ? (256 / BK) * sr : (256 / ROWS) * sk) * (int64_t)sizeof(T)));
+    }
+    // the slab fits the resource: rows R0 .. R0 + ROWS - 1 exist and all offsets stay below 2^31
+    static __device__ __forceinline__ bool fits(int64_t sr, int64_t sk, int R0, int R, int K) {
+        return R0 + ROWS <= R && sr >= 0 && sk >= 0 &&
+               ((int64_t)(ROWS - 1) * sr + (int64_t)(K - 1) * sk) * (int64_t)sizeof(T) < ((int64_t)1 << 31);
+    }
+    __device__ __forceinline__ void load(int k0, double (&reg)[ROWS * BK / 256]) const {
+        const uint32_t vo = voff0 + (uint32_t)k0 * kbytes;
+#pragma unroll
+        for (int s = 0; s < ROWS * BK / 256; ++s) {
+            if constexpr (sizeof(T) == 8)
+                reg[s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)vo, (int)(s * sbytes), 0));
+            else
+                reg[s] = (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)vo, (int)(s * sbytes), 0));
+        }
+    }
+};
+
 template <int ROWS, int BK>
 __device__ __forceinline__ void panel_coord(int s, bool kc, int &r, int &k) {
     int e = threadIdx.x + 256 * s;
@@ -143,49 +196,57 @@ __global__ void __launch_bounds__(256) k_gemm_fp(GemmArgs args) {
     for (int i = 0; i < TI; ++i)
 #pragma unroll
         for (int j = 0; j < TJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-    double ra[NA], rb[NB];
     const bool akc = (g.sak == 1), bkc = (g.sbk == 1);
-    if (K > 0) {
-        load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, 0, g.M, K, ra);
-        load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, 0, g.N, K, rb);
-    }
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < NA; ++s) {
-            int r, k;
-            panel_coord<BM, BK>(s, akc, r, k);
-            As[k][r] = (TC)ra[s];
+    // slot s of this lane holds panel element (r + s dr, k + s dk) and goes to Xs[k + s dk][r + s dr]
+    const int t = threadIdx.x;
+    const int lda0 = akc ? (t % BK) * (BM + PAD) + t / BK : (t / BM) * (BM + PAD) + t % BM;
+    const int ldb0 = bkc ? (t % BK) * (BN + PAD) + t / BK : (t / BN) * (BN + PAD) + t % BN;
+    const int lda_step = akc ? 256 / BK : (256 / BM) * (BM + PAD);
+    const int ldb_step = bkc ? 256 / BK : (256 / BN) * (BN + PAD);
+
+    auto mainloop = [&](const auto &la, const auto &lb) {
+        double ra[NA], rb[NB];
+        if (K > 0) {
+            la.load(0, ra);
+            lb.load(0, rb);
         }
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            __syncthreads();
 #pragma unroll
-        for (int s = 0; s < NB; ++s) {
-            int r, k;
-            panel_coord<BN, BK>(s, bkc, r, k);
-            Bs[k][r] = (TC)rb[s];
+            for (int s = 0; s < NA; ++s) (&As[0][0])[lda0 + s * lda_step] = (TC)ra[s];
+#pragma unroll
+            for (int s = 0; s < NB; ++s) (&Bs[0][0])[ldb0 + s * ldb_step] = (TC)rb[s];
+            __syncthreads();
+            if (k0 + BK < K) {  // prefetch the next panels while the matrix cores work
+                la.load(k0 + BK, ra);
+                lb.load(k0 + BK, rb);
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                TC af[TI], bf[TJ];
+#pragma unroll
+                for (int i = 0; i < TI; ++i) af[i] = As[kk + lane / 16][wm + 16 * i + lane % 16];
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) bf[j] = Bs[kk + lane / 16][wn + 16 * j + lane % 16];
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) {
+                        if constexpr (sizeof(TC) == 8)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                        else
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    }
+            }
         }
-        __syncthreads();
-        if (k0 + BK < K) {  // prefetch the next panels while the matrix cores work
-            load_panel<TIN, BM, BK>(A, g.sam, g.sak, m0, k0 + BK, g.M, K, ra);
-            load_panel<TIN, BN, BK>(B, g.sbn, g.sbk, n0, k0 + BK, g.N, K, rb);
-        }
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            TC af[TI], bf[TJ];
-#pragma unroll
-            for (int i = 0; i < TI; ++i) af[i] = As[kk + lane / 16][wm + 16 * i + lane % 16];
-#pragma unroll
-            for (int j = 0; j < TJ; ++j) bf[j] = Bs[kk + lane / 16][wn + 16 * j + lane % 16];
-#pragma unroll
-            for (int i = 0; i < TI; ++i)
-#pragma unroll
-                for (int j = 0; j < TJ; ++j) {
-                    if constexpr (sizeof(TC) == 8)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-                    else
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
-                }
-        }
-    }
+    };
+    // interior tiles with whole panels take the buffer-load walk, the others the range-checked one
+    if (K % BK == 0 && InnerPanels<TIN, BM, BK>::fits(g.sam, g.sak, m0, g.M, K) &&
+        InnerPanels<TIN, BN, BK>::fits(g.sbn, g.sbk, n0, g.N, K))
+        mainloop(InnerPanels<TIN, BM, BK>(A, g.sam, g.sak, m0), InnerPanels<TIN, BN, BK>(B, g.sbn, g.sbk, n0));
+    else
+        mainloop(CheckedPanels<TIN, BM, BK>{A, g.sam, g.sak, m0, g.M, K},
+                 CheckedPanels<TIN, BN, BK>{B, g.sbn, g.sbk, n0, g.N, K});
     const double alpha = g.alpha, beta = g.beta;
 #pragma unroll
     for (int i = 0; i < TI; ++i)

@@ -113,6 +113,33 @@ inline int __builtin_amdgcn_readlane(int v, int l) {
     return r;
 }
 
+// value of v in the wavefront's first lane (every lane is active in the emulation)
+inline int __builtin_amdgcn_readfirstlane(int v) { return __builtin_amdgcn_readlane(v, 0); }
+
+// raw buffer resource (stride 0): loads at byte offset voffset + soffset from the base; an offset
+// at or beyond num_records reads zero, as the hardware's range check does
+struct emu_buffer_rsrc {
+    const char *base;
+    unsigned num_records;
+};
+typedef emu_buffer_rsrc __amdgpu_buffer_rsrc_t;
+inline emu_buffer_rsrc __builtin_amdgcn_make_buffer_rsrc(void *p, short, int num_records, int) {
+    return emu_buffer_rsrc{(const char *)p, (unsigned)num_records};
+}
+template <typename T>
+inline T emu_buffer_load(emu_buffer_rsrc r, int voffset, int soffset) {
+    const unsigned o = (unsigned)voffset + (unsigned)soffset;
+    T v = 0;
+    if ((unsigned long long)o + sizeof(T) <= r.num_records) memcpy(&v, r.base + o, sizeof(T));
+    return v;
+}
+inline unsigned long long __builtin_amdgcn_raw_buffer_load_b64(emu_buffer_rsrc r, int vo, int so, int) {
+    return emu_buffer_load<unsigned long long>(r, vo, so);
+}
+inline unsigned __builtin_amdgcn_raw_buffer_load_b32(emu_buffer_rsrc r, int vo, int so, int) {
+    return emu_buffer_load<unsigned>(r, vo, so);
+}
+
 // value of v in lane src (src may differ per lane; lanes outside [0, width) read their own value)
 template <typename T>
 inline T __shfl(T v, int src, int width = 64) {

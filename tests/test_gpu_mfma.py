@@ -92,3 +92,46 @@ def test_int8_mfma_lane_layout():
     if not np.array_equal(got, exp):
         print("device result registers (lane x register):\n", got)
     np.testing.assert_array_equal(got, exp)
+
+
+def _layout_case(rng, M, N, K, batch, a_rowmajor, b_rowmajor):
+    """Operands of one FP64 group with A stored (M, K) or (K, M) and B stored (K, N) or (N, K)."""
+    A = rng.standard_normal((batch, M, K) if a_rowmajor else (batch, K, M))
+    B = rng.standard_normal((batch, K, N) if b_rowmajor else (batch, N, K))
+    Aop = A if a_rowmajor else A.transpose(0, 2, 1)
+    Bop = B if b_rowmajor else B.transpose(0, 2, 1)
+    return A, B, Aop @ Bop
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5)])
+@pytest.mark.parametrize("tile_launch", ["128", "64"])
+def test_gemm_interior_and_edge_panels(prec, tol, tile_launch):
+    """Tiles whose operand slabs lie wholly in range (rows inside, K a multiple of the panel depth)
+    read their panels through buffer loads at per-slot offsets; edge tiles and ragged reductions
+    take the range-checked loads.  Each launch mixes both kinds over all four storage orders of A and
+    B (beta = 1 on a zero C, so the launch follows the tile heuristics: 128 x 128 tiles when every
+    group is at least 128 x 128 and the launch has 256 such tiles), against numpy."""
+    from grassmanntn_amd import _lib as L
+    from grassmanntn_amd.core import _gemm_desc, _gemm
+    from grassmanntn_amd._device import DeviceArray
+    rng = np.random.default_rng(23)
+    if tile_launch == "128":  # 4 x 4 x 8 + 2 x 2 x 2 x 6 + ... >= 256 tiles of 128 x 128
+        cases = [(512, 512, 32, 8, True, True), (256, 256, 48, 6, False, True), (256, 384, 32, 8, True, False),
+                 (200, 136, 40, 2, False, False), (384, 256, 16, 8, True, True)]
+    else:
+        cases = [(192, 128, 48, 2, True, True), (128, 192, 32, 2, False, False), (100, 72, 24, 1, True, False),
+                 (64, 64, 16, 3, False, True)]
+    groups, keep = [], []
+    for (M, N, K, batch, arow, brow) in cases:
+        A, B, ref = _layout_case(rng, M, N, K, batch, arow, brow)
+        dA, dB = DeviceArray.from_numpy(A), DeviceArray.from_numpy(B)
+        dC = DeviceArray.zeros((batch, M, N))
+        sam, sak = (K, 1) if arow else (1, M)
+        sbk, sbn = (N, 1) if brow else (1, K)
+        groups.append(_gemm_desc(dA.data_ptr(), sam, sak, dB.data_ptr(), sbk, sbn, dC.data_ptr(), N, 1, M, N, K,
+                                 batch=batch, sab=M * K, sbb=K * N, scb=M * N, alpha=1.0, beta=1.0))
+        keep.append((dC, ref, dA, dB))  # the operands stay allocated until the launch has run
+    _gemm(groups, L.dtype_code(keep[0][0]), prec)
+    for i, (dC, ref, _a, _b) in enumerate(keep):
+        err = np.abs(dC.numpy() - ref).max() / np.abs(ref).max()
+        assert err <= tol, (cases[i], prec, err)
