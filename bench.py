@@ -240,8 +240,16 @@ def rf_gemm_roofline(P, reps=20):
     from grassmanntn_amd._device import DeviceArray, Event, synchronize
     from grassmanntn_amd.core import _gemm_desc
     nb, m, k, p = 4 * P, D * D // 2, D * D // 2, 128
-    A = DeviceArray.zeros((nb, m, k))
-    B = DeviceArray.zeros((nb, k, p))
+    # operands of N(0, 1) entries (one seeded block repeated): the matrix cores' clock under load
+    # depends on the data they multiply, and zero operands would flatter the kernel
+    rng = np.random.default_rng(SEED)
+    a1 = DeviceArray.from_numpy(rng.standard_normal((m, k)))
+    b1 = DeviceArray.from_numpy(rng.standard_normal((k, p)))
+    A = DeviceArray.empty((nb, m, k))
+    B = DeviceArray.empty((nb, k, p))
+    for i in range(nb):
+        A.flat(i * m * k, m * k, shape=(m, k)).copy_from(a1)
+        B.flat(i * k * p, k * p, shape=(k, p)).copy_from(b1)
     Cm = DeviceArray.empty((nb, m, p))
     d = _gemm_desc(A.data_ptr(), k, 1, B.data_ptr(), p, 1, Cm.data_ptr(), p, 1, m, p, k, batch=nb,
                    sab=m * k, sbb=k * p, scb=m * p)
@@ -259,10 +267,16 @@ def rf_gemm_roofline(P, reps=20):
     ms = e0.elapsed_ms(e1) / reps
     flop = 2.0 * m * p * k * nb
     ach = flop / (ms / 1000.0) / 1e12
+    # the launch configuration the autotuner settled on for this signature (first calls above)
+    tile = 128 if nb * (m // 128) >= 256 else 64
+    rep = (L.TuneEntry * 256)()
+    for e in rep[:min(lib.gtn_autotune_report(rep, 256), 256)]:
+        if (e.M, e.N, e.K, e.batch, e.n_groups, e.precision) == (m, p, k, nb, 1, L.PREC["f64"]):
+            tile = e.tile
     return {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TF["f64"], "unit": "TFLOP/s",
             "frac": ach / MFMA_PEAK_TF["f64"], "traffic": measured_traffic("rf_gemm_f64"),
-            "kernel": "range-finder GEMM k_gemm_fp<double,double,128,128,16>: %d parity blocks of 2048 x 2048 "
-                      "times 2048 x 128 per launch, FP64" % nb,
+            "kernel": "range-finder GEMM k_gemm_fp<double,double,%d,%d,16>: %d parity blocks of 2048 x 2048 "
+                      "times 2048 x 128 (N(0,1) entries) per launch, FP64" % (tile, tile, nb),
             "flop_per_launch": flop, "avg_launch_ms": ms, "launches": reps}
 
 

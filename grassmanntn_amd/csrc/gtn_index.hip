@@ -199,6 +199,46 @@ __global__ void k_perm_tables(PermArgs a, TabBuild t, TabEntry *out) {
     }
 }
 
+// Table indices of the output axes 0 .. hi for the row-major index `lin` over those axes.
+__device__ __forceinline__ void tab_index(const PermArgs &a, const TabArgs &t, uint64_t lin, int hi,
+                                          uint32_t (&G)[kTabGroups]) {
+#pragma unroll
+    for (int g = 0; g < kTabGroups; ++g) G[g] = 0;
+    uint64_t rest = lin;
+    for (int ax = hi; ax >= 0; --ax) {
+        uint32_t o;
+        if (a.out_shift[ax] >= 0) {
+            o = (uint32_t)(rest & (uint64_t)(a.out_dim[ax] - 1));
+            rest >>= a.out_shift[ax];
+        } else if ((rest >> 32) == 0) {
+            const uint32_t r32 = (uint32_t)rest, d = a.out_dim[ax];
+            o = r32 % d;
+            rest = r32 / d;
+        } else {
+            o = (uint32_t)(rest % a.out_dim[ax]);
+            rest /= a.out_dim[ax];
+        }
+#pragma unroll
+        for (int g = 0; g < kTabGroups; ++g) G[g] += o * t.gmul[ax][g];
+    }
+}
+
+// Source offset and sign code of the element with table indices G.
+__device__ __forceinline__ int64_t tab_lookup(const PermArgs &a, const TabArgs &t, const uint32_t (&G)[kTabGroups],
+                                              uint32_t &code) {
+    int64_t off = 0;
+    uint32_t c = 0;
+#pragma unroll
+    for (int g = 0; g < kTabGroups; ++g)
+        if (g < t.ng) {
+            const TabEntry e = t.tab[g][G[g]];
+            off += e.off;
+            c ^= e.code;
+        }
+    code = sign_code(a, c & 0x7fffffffu, c >> 31);
+    return off;
+}
+
 __device__ __forceinline__ int64_t tab_map(const PermArgs &a, const TabArgs &t, uint64_t lin, uint32_t &code) {
     uint32_t G[kTabGroups];
 #pragma unroll
@@ -247,10 +287,59 @@ __device__ __forceinline__ int64_t map_elem(const PermArgs &a, const TabArgs &t,
 // kPermUnroll elements together, then their source reads, so kPermUnroll dependent load chains are in
 // flight per lane instead of one.
 constexpr int kPermUnroll = 4;
+// Output suffix for the chunk-uniform walk: the fewest trailing output axes (at most two) whose
+// extent is a multiple of the chunk, so that every chunk shares the indices of the axes before them;
+// 0 when there is none.
+__device__ __forceinline__ int chunk_suffix(const PermArgs &a, int64_t chunk) {
+    if (a.n_out >= 1 && (int64_t)a.out_dim[a.n_out - 1] % chunk == 0) return 1;
+    if (a.n_out >= 2 && a.out_shift[a.n_out - 1] >= 0 &&
+        ((int64_t)a.out_dim[a.n_out - 1] * a.out_dim[a.n_out - 2]) % chunk == 0)
+        return 2;
+    return 0;
+}
+
 template <typename T, bool TAB>
 __global__ void __launch_bounds__(256) k_permute(PermArgs a, TabArgs tb, const T *__restrict__ src, T *__restrict__ dst) {
     const double al = a.alpha_re;
     const int64_t chunk = (int64_t)kPermUnroll * blockDim.x;
+    if constexpr (TAB) {
+        // chunk-uniform walk: the leading axes' table indices are formed once per chunk (the same in
+        // every lane), each element adds only its trailing one or two indices times their multipliers
+        const int ns = chunk_suffix(a, chunk);
+        if (ns > 0) {
+            const int last = a.n_out - 1;
+            const uint64_t span = ns == 1 ? a.out_dim[last] : (uint64_t)a.out_dim[last] * a.out_dim[last - 1];
+            for (int64_t i0 = blockIdx.x * chunk; i0 < a.total; i0 += (int64_t)gridDim.x * chunk) {
+                const uint64_t row = (uint64_t)i0 / span;
+                const uint32_t c0 = (uint32_t)((uint64_t)i0 - row * span);
+                uint32_t Gb[kTabGroups];
+                tab_index(a, tb, row, last - ns, Gb);
+                int64_t off[kPermUnroll];
+                uint32_t code[kPermUnroll];
+#pragma unroll
+                for (int u = 0; u < kPermUnroll; ++u) {
+                    const uint32_t c = c0 + threadIdx.x + u * blockDim.x;
+                    uint32_t o0 = c, o1 = 0;
+                    if (ns == 2) {
+                        o0 = c & (a.out_dim[last] - 1);
+                        o1 = c >> a.out_shift[last];
+                    }
+                    uint32_t G[kTabGroups];
+#pragma unroll
+                    for (int g = 0; g < kTabGroups; ++g)
+                        G[g] = Gb[g] + o0 * tb.gmul[last][g] + (ns == 2 ? o1 * tb.gmul[last - 1][g] : 0u);
+                    off[u] = tab_lookup(a, tb, G, code[u]);
+                }
+                double v[kPermUnroll];
+#pragma unroll
+                for (int u = 0; u < kPermUnroll; ++u) v[u] = (code[u] == 2u) ? 0.0 : (double)src[off[u]];
+#pragma unroll
+                for (int u = 0; u < kPermUnroll; ++u)
+                    dst[i0 + threadIdx.x + u * blockDim.x] = (T)(code[u] == 2u ? 0.0 : v[u] * (code[u] ? -al : al));
+            }
+            return;
+        }
+    }
     for (int64_t i0 = blockIdx.x * chunk + threadIdx.x; i0 < a.total; i0 += (int64_t)gridDim.x * chunk) {
         int64_t off[kPermUnroll];
         uint32_t code[kPermUnroll];
