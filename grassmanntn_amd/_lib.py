@@ -102,6 +102,7 @@ def _bind(lib):
         "gtn_device_info": (C.c_int, [C.c_char_p, i32, C.POINTER(C.c_int)]),
         "gtn_malloc": (C.c_int, [i64, C.POINTER(C.c_void_p)]),
         "gtn_free": (C.c_int, [vp]),
+        "gtn_mem_info": (C.c_int, [C.POINTER(C.c_int64)] * 4),
         "gtn_memcpy": (C.c_int, [vp, vp, i64, i32, vp]),
         "gtn_fill": (C.c_int, [vp, i64, i32, d, vp]),
         "gtn_copy2d": (C.c_int, [vp, i64, vp, i64, i64, i64, i32, vp]),
@@ -218,3 +219,11 @@ def dtype_code(t):
     if dt == np.int8:
         return GTN_I8
     raise GrassmannDeviceError("unsupported dtype %s" % dt)
+
+
+def mem_info():
+    """{live, cached, device_free, device_total}: bytes held by live allocations, bytes the allocator
+    keeps for reuse, and the device's free and total memory (gtn_mem_info)."""
+    v = [C.c_int64(0) for _ in range(4)]
+    check(lib().gtn_mem_info(*[C.byref(x) for x in v]), "gtn_mem_info")
+    return dict(zip(("live", "cached", "device_free", "device_total"), (x.value for x in v)))

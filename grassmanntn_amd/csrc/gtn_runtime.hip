@@ -142,6 +142,24 @@ int gtn_free(void *p) {
     return GTN_OK;
 }
 
+int gtn_mem_info(int64_t *live_bytes, int64_t *cached_bytes, int64_t *device_free, int64_t *device_total) {
+    int64_t live = 0, cached = 0;
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        for (auto &kv : g_live) live += kv.second.cls;
+        cached = g_cached_bytes;
+        for (auto &pv : g_pools)
+            for (auto &kv : pv.second.free) cached += kv.first * (int64_t)kv.second.size();
+    }
+    size_t f = 0, t = 0;
+    GTN_CHECK_HIP(hipMemGetInfo(&f, &t));
+    if (live_bytes) *live_bytes = live;
+    if (cached_bytes) *cached_bytes = cached;
+    if (device_free) *device_free = (int64_t)f;
+    if (device_total) *device_total = (int64_t)t;
+    return GTN_OK;
+}
+
 // kind: 1 host->device, 2 device->host, 3 device->device.  Host<->device copies are synchronous;
 // device->device copies are ordered on `stream`.
 int gtn_memcpy(void *dst, const void *src, int64_t bytes, int kind, void *stream) {

@@ -70,9 +70,15 @@ class CouplingSweep:
         self.runs = {}
 
     def _default_batch(self, chi):
+        """Points per batch at bond chi: a third of the device's memory (at least 24 GiB) over an
+        upper estimate of one point's step buffers, eager and recorded (24 arrays of d^4 FP64 values
+        at bond d; measured on MI355X: 2.1 GB per point at leg dim 32 -> bond 64, 0.12 GB at bond 32),
+        at most 512."""
         d = _bond(min(self.D * self.D // 2, int(chi / 2)))
-        per_point = 24 * 8 * max(d, self.D) ** 4   # step buffers of one point, eager and recorded
-        return int(max(1, min(512, (24 << 30) // per_point)))
+        per_point = 24 * 8 * max(d, self.D) ** 4
+        if not hasattr(self, "_budget"):
+            self._budget = max(24 << 30, L.mem_info()["device_total"] // 3)
+        return int(max(1, min(512, self._budget // per_point)))
 
     def _sites(self, coef):
         """(P, D^4) device array of A0 + beta A1 + m A2 for coef rows (1, beta, m): one GEMM."""
@@ -85,18 +91,31 @@ class CouplingSweep:
         return out
 
     def batches(self, beta, m, chi):
-        """[(chi, batch size, indices of the points of one batch)]: points grouped by chi, at most
-        max_batch(chi) per batch; a batch's size is the smallest power of two (or the maximum) that
-        holds its points, so that few sizes -- and few recorded graphs -- serve any number of points."""
+        """[(chi, batch size, indices of the points of one batch)]: points grouped by chi, spread
+        evenly over the fewest batches of at most max_batch(chi) points; the batch size is that
+        count rounded up to a multiple of max(8, q / 16), q the next power of two (or the maximum):
+        at most 16 sizes per octave -- few recorded graphs serve any number of points -- and at most
+        1/16 of a batch is padding."""
         out = []
         for c in sorted(set(int(x) for x in chi)):
             idx = np.nonzero(np.asarray(chi) == c)[0]
             B = self.max_batch(c)
-            for s in range(0, len(idx), B):
-                part = idx[s:s + B]
-                P = min(B, 1 << int(np.ceil(np.log2(len(part)))))
-                out.append((c, P, part))
+            nb = -(-len(idx) // B)          # batches needed, then points spread evenly over them so
+            per = -(-len(idx) // nb)        # that they share one size (and one set of graphs)
+            P = min(B, self.batch_size(per))
+            for s in range(0, len(idx), per):
+                out.append((c, P, idx[s:s + per]))
         return out
+
+    @staticmethod
+    def batch_size(n):
+        """Slots of a batch holding n points: n rounded up to a multiple of max(8, q / 16) with q the
+        smallest power of two >= n (powers of two up to 8 stay as they are)."""
+        q = 1 << int(np.ceil(np.log2(max(1, n))))
+        if q <= 8:
+            return q
+        g = max(8, q // 16)
+        return -(-n // g) * g
 
     def _run_for(self, chi, P, beta=None, m=None, idx=None):
         key = (int(chi), int(P))

@@ -75,6 +75,11 @@ int gtn_device_info(char *name, int name_len, int *n_cu);
  * ---------------------------------------------------------------------------------------------- */
 int gtn_malloc(int64_t bytes, void **out_dev);
 int gtn_free(void *dev);
+/* Bytes the allocator has handed out and not taken back (live), bytes it holds for reuse (cached:
+ * the general cache and the free blocks of graph pools), and the device's free / total memory
+ * (hipMemGetInfo).  No counterpart in the reference, which keeps its arrays on the host; callers use
+ * it to size batches. */
+int gtn_mem_info(int64_t *live_bytes, int64_t *cached_bytes, int64_t *device_free, int64_t *device_total);
 int gtn_memcpy(void *dst, const void *src, int64_t bytes, int kind, void *stream);
 int gtn_fill(void *dev, int64_t n, int dtype, double value, void *stream);
 int gtn_copy2d(void *dst_dev, int64_t ldd, const void *src_dev, int64_t lds, int64_t rows, int64_t cols, int dtype,

@@ -89,6 +89,11 @@ class FakeLib:
     def gtn_set_dry_run(self, n, bases, sizes):
         return 0
 
+    def gtn_mem_info(self, live, cached, free, total):
+        for ref, v in ((live, 0), (cached, 0), (free, 64 << 30), (total, 64 << 30)):
+            ref._obj.value = v
+        return 0
+
     # memory pools as the runtime keeps them: a block allocated while a capture is open belongs to
     # that capture's pool; freeing it returns it to the pool (it stays allocated, since a recorded
     # graph may address it) for reuse by later allocations of the same pool
@@ -583,7 +588,7 @@ class FakeLib:
         return 0
 
 
-_NOT_RECORDED = {"gtn_malloc", "gtn_free", "gtn_version", "gtn_last_error", "gtn_device_info", "gtn_set_dry_run",
+_NOT_RECORDED = {"gtn_malloc", "gtn_free", "gtn_mem_info", "gtn_version", "gtn_last_error", "gtn_device_info", "gtn_set_dry_run",
                  "gtn_mlp_colsum_workspace", "gtn_autotune_set", "gtn_autotune_clear", "gtn_autotune_report",
                  "gtn_svd_workspace_bytes", "gtn_gemm_noalloc_max_scratch", "gtn_event_create", "gtn_event_destroy",
                  "gtn_stream_create", "gtn_stream_destroy", "gtn_graph_begin", "gtn_graph_end", "gtn_graph_launch",
