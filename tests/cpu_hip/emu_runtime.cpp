@@ -103,6 +103,11 @@ hipError_t hipMalloc(void **p, size_t n) {
     *p = malloc(n);
     return hipSuccess;
 }
+// a nominal 64 GiB device, all of it free (the emulation does not track host allocations)
+hipError_t hipMemGetInfo(size_t *free_bytes, size_t *total_bytes) {
+    *free_bytes = *total_bytes = (size_t)64 << 30;
+    return hipSuccess;
+}
 hipError_t hipGetDevice(int *d) {
     *d = 0;
     return hipSuccess;

@@ -225,6 +225,7 @@ inline emu_i4 __builtin_amdgcn_mfma_i32_16x16x64_i8(emu_i4 a, emu_i4 b, emu_i4 c
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t);
 hipError_t hipMalloc(void **p, size_t n);
+hipError_t hipMemGetInfo(size_t *free_bytes, size_t *total_bytes);
 template <typename T>
 inline hipError_t hipMalloc(T **p, size_t n) {
     return hipMalloc((void **)p, n);
