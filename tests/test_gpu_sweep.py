@@ -30,3 +30,21 @@ def test_sweep_subset_matches_oracle():
     for p, e in zip(g["points"], err):
         print("chi %d point %d |dlnZ/site| %.2e" % (p["chi"], p["index"], e))
     assert err.max() <= 1e-8
+
+
+def test_memory_report_and_default_batch():
+    """gtn_mem_info counts a fresh allocation as live and reports the device's memory; the sweep's
+    default batch at bond 64 (leg-dim-32 generators, chi 48 and 64) is a third of that memory over
+    24 x 64^4 FP64 values per point."""
+    from grassmanntn_amd import _lib as L
+    from grassmanntn_amd._device import DeviceArray
+    from grassmanntn_amd.sweep import CouplingSweep
+    m0 = L.mem_info()
+    x = DeviceArray.empty((8 << 20,))  # 64 MiB
+    m1 = L.mem_info()
+    assert m1["live"] - m0["live"] >= x.nbytes
+    assert 0 < m1["device_free"] <= m1["device_total"]
+    del x
+    sw = CouplingSweep(D=32, steps=2)
+    want = max(1, min(512, max(24 << 30, m1["device_total"] // 3) // (24 * 8 * 64 ** 4)))
+    assert sw.max_batch(64) == sw.max_batch(48) == want
