@@ -288,13 +288,14 @@ __device__ __forceinline__ int64_t map_elem(const PermArgs &a, const TabArgs &t,
 // flight per lane instead of one.
 constexpr int kPermUnroll = 4;
 // Output suffix for the chunk-uniform walk: the fewest trailing output axes (at most two) whose
-// extent is a multiple of the chunk, so that every chunk shares the indices of the axes before them;
-// 0 when there is none.
+// extent is a multiple of the chunk and below 2^32 (the in-suffix index is 32-bit), so that every
+// chunk shares the indices of the axes before them; 0 when there is none.
 __device__ __forceinline__ int chunk_suffix(const PermArgs &a, int64_t chunk) {
     if (a.n_out >= 1 && (int64_t)a.out_dim[a.n_out - 1] % chunk == 0) return 1;
-    if (a.n_out >= 2 && a.out_shift[a.n_out - 1] >= 0 &&
-        ((int64_t)a.out_dim[a.n_out - 1] * a.out_dim[a.n_out - 2]) % chunk == 0)
-        return 2;
+    if (a.n_out >= 2 && a.out_shift[a.n_out - 1] >= 0) {
+        const int64_t span = (int64_t)a.out_dim[a.n_out - 1] * a.out_dim[a.n_out - 2];
+        if (span % chunk == 0 && span < ((int64_t)1 << 32)) return 2;
+    }
     return 0;
 }
 
