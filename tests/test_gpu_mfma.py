@@ -94,23 +94,26 @@ def test_int8_mfma_lane_layout():
     np.testing.assert_array_equal(got, exp)
 
 
-def _layout_case(rng, M, N, K, batch, a_rowmajor, b_rowmajor):
-    """Operands of one FP64 group with A stored (M, K) or (K, M) and B stored (K, N) or (N, K)."""
-    A = rng.standard_normal((batch, M, K) if a_rowmajor else (batch, K, M))
-    B = rng.standard_normal((batch, K, N) if b_rowmajor else (batch, N, K))
+def _layout_case(rng, M, N, K, batch, a_rowmajor, b_rowmajor, dt=np.float64):
+    """Operands (element type dt) of one group with A stored (M, K) or (K, M) and B stored (K, N) or
+    (N, K), and their FP64 product."""
+    A = rng.standard_normal((batch, M, K) if a_rowmajor else (batch, K, M)).astype(dt)
+    B = rng.standard_normal((batch, K, N) if b_rowmajor else (batch, N, K)).astype(dt)
     Aop = A if a_rowmajor else A.transpose(0, 2, 1)
     Bop = B if b_rowmajor else B.transpose(0, 2, 1)
-    return A, B, Aop @ Bop
+    return A, B, Aop.astype(np.float64) @ Bop.astype(np.float64)
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 1e-5)])
+@pytest.mark.parametrize("prec,tol,dt", [("f64", 1e-13, np.float64), ("f32", 1e-5, np.float64),
+                                         ("f32", 1e-5, np.float32), ("f64", 1e-5, np.float32)])
 @pytest.mark.parametrize("tile_launch", ["128", "64"])
-def test_gemm_interior_and_edge_panels(prec, tol, tile_launch):
+def test_gemm_interior_and_edge_panels(prec, tol, dt, tile_launch):
     """Tiles whose operand slabs lie wholly in range (rows inside, K a multiple of the panel depth)
     read their panels through buffer loads at per-slot offsets; edge tiles and ragged reductions
     take the range-checked loads.  Each launch mixes both kinds over all four storage orders of A and
     B (beta = 1 on a zero C, so the launch follows the tile heuristics: 128 x 128 tiles when every
-    group is at least 128 x 128 and the launch has 256 such tiles), against numpy."""
+    group is at least 128 x 128 and the launch has 256 such tiles), for FP64 and FP32 data, against
+    numpy."""
     from grassmanntn_amd import _lib as L
     from grassmanntn_amd.core import _gemm_desc, _gemm
     from grassmanntn_amd._device import DeviceArray
@@ -123,9 +126,9 @@ def test_gemm_interior_and_edge_panels(prec, tol, tile_launch):
                  (64, 64, 16, 3, False, True)]
     groups, keep = [], []
     for (M, N, K, batch, arow, brow) in cases:
-        A, B, ref = _layout_case(rng, M, N, K, batch, arow, brow)
+        A, B, ref = _layout_case(rng, M, N, K, batch, arow, brow, dt)
         dA, dB = DeviceArray.from_numpy(A), DeviceArray.from_numpy(B)
-        dC = DeviceArray.zeros((batch, M, N))
+        dC = DeviceArray.zeros((batch, M, N), dt)
         sam, sak = (K, 1) if arow else (1, M)
         sbk, sbn = (N, 1) if brow else (1, K)
         groups.append(_gemm_desc(dA.data_ptr(), sam, sak, dB.data_ptr(), sbk, sbn, dC.data_ptr(), N, 1, M, N, K,
